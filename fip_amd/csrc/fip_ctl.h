@@ -1,0 +1,69 @@
+// Device-resident control block of one simulation: time, dt, iteration, ping-pong
+// side, CFL accumulators, the active-tile schedule and the host-event flags. It lives in
+// device memory and is shared by the precompiled kernels and the run-time compiled
+// boundary-condition kernel (this file is also handed to hipRTC as a header).
+#pragma once
+
+#define FIP_MAX_BC 256
+#define FIP_MAX_SAMPLE_POINTS 64
+#define FIP_SAMPLE_LOG_ROWS 256
+#define FIP_MAX_SAVE 256
+
+// Stop requests raised by the post-step kernel (bits).
+#define FIP_HALT_FINISHED 1  // duration >= target: the loop of src/main.cpp:243 ends
+#define FIP_HALT_DUMP 2      // a save_state time was reached: the host dumps, then resumes
+
+typedef struct fip_ctl {
+  double duration;       // simulated time (src/simdata.cpp:19), double as in the reference
+  double target;         // target_duration
+  double sample_timer;   // Sampling::timer
+  double save_times[FIP_MAX_SAVE];  // save_state times, sorted (src/simdata.cpp:304-309)
+  float dt;              // current timestep
+  float cfl_prev;        // CFL minimum written by the last stencil step (starts at 1)
+  unsigned cfl_acc[2];   // float bits; stencil of iteration n min-reduces into [n & 1]
+  int iteration;         // the current ("1") buffers are index iteration & 1
+  // Stop requests. The post-step kernel of a step with parity p = iteration & 1 writes
+  // halt_req[p]; every kernel of the next step (parity p^1) reads halt_req[p] and does
+  // nothing when it is set, and a skipped post kernel copies the request forward. No
+  // kernel reads a word that another block of the same launch writes.
+  int halt_req[2];
+  int variable_dt;
+  int sampling_interval; // seconds; 0 = off
+  int nsave;             // number of valid entries in save_times
+  int save_pos;          // next entry of save_times
+  int nsample_rows;      // rows written to the sample log since the host last drained it
+  int nsample_points;
+  int sample_overflow;
+  // active-tile schedule (masked mode), double-buffered by iteration parity
+  unsigned list_count[2];
+  unsigned work_next[2];
+  // boundary-condition parameters of the last BC application
+  float bc_q[FIP_MAX_BC];
+  float bc_z[FIP_MAX_BC];
+  float swd_z[FIP_MAX_BC];    // sumWaterDepths cache: last z and its value
+  float swd_val[FIP_MAX_BC];
+  int swd_valid[FIP_MAX_BC];
+  // sample points and the log of sampled rows
+  int sp_x[FIP_MAX_SAMPLE_POINTS];
+  int sp_y[FIP_MAX_SAMPLE_POINTS];
+  int sp_type[FIP_MAX_SAMPLE_POINTS];  // 0=z 1=h 2=qx 3=qy
+  double sample_t[FIP_SAMPLE_LOG_ROWS];
+  float sample_v[FIP_SAMPLE_LOG_ROWS * FIP_MAX_SAMPLE_POINTS];
+} fip_ctl;
+
+// Pointers and sizes of one simulation, passed by value to every kernel.
+typedef struct fip_grid {
+  float* h[2];
+  float* qx[2];
+  float* qy[2];
+  float* hmax;
+  const float* bed;
+  const float* kst;       // NULL when kSt is uniform
+  unsigned long long* mask;   // one 64-bit row mask per tile
+  unsigned* list[2];          // active tiles: (tile << 12) | (lo << 6) | hi
+  const float* series;        // BC time-series tables
+  fip_ctl* ctl;
+  int W, H, P;            // P = row stride in floats; element (i, j) is at [j * P + i]
+  int tiles_x, tiles_y, tile_rows;
+  float dx, no_data, kst_uniform;
+} fip_grid;

@@ -1,0 +1,928 @@
+// libfiphip: host side of the C ABI (include/fip.h). Allocation and upload, the hipRTC
+// boundary-kernel generator, the fine-grained step seam and the device-resident
+// simulate() loop with hipGraph-captured chunks.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fip.h"
+#include "fip_ctl.h"
+#include "fip_embed.h"  // generated: the device headers as strings for hipRTC
+
+hipError_t fip_launch_stencil_exact(const fip_grid& g, int par, int masked, int blocks,
+                                    hipStream_t st);
+hipError_t fip_launch_stencil_fast(const fip_grid& g, int par, int masked, int blocks,
+                                   hipStream_t st);
+hipError_t fip_launch_misc(int which, const fip_grid& g, int arg, int blocks, hipStream_t st);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPTRY(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(FIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+constexpr int kTileCols = 60;
+constexpr int kMarginCols = 64;
+constexpr int kMarginRows = 8;
+constexpr int kStepsPerGraph = 100;  // a multiple of 2 and the flood-plain cadence
+
+struct BCDesc {
+  int type, side, from, to;
+  float z, q;
+  int zoff, zn, qoff, qn;
+};
+
+std::string hexf(float v) {
+  if (std::isnan(v)) return "__builtin_nanf(\"\")";
+  if (std::isinf(v)) return v > 0 ? "__builtin_inff()" : "(-__builtin_inff())";
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%af", (double)v);
+  return buf;
+}
+
+}  // namespace
+
+struct fip_sim {
+  fip_desc d{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  fip_grid g{};
+  float* base_alloc[9] = {};
+  size_t field_elems = 0;  // elements per allocation (with margins)
+  long interior_off = 0;   // offset of element (0,0)
+  int ntiles = 0;
+  int host_iter = 0;
+  bool masked = false;
+  bool fast = false;
+  int stencil_blocks = 0;
+  int sched_blocks = 0;
+  // boundary kernel
+  hipModule_t module = nullptr;
+  hipFunction_t bc_fn = nullptr;
+  int nbc = 0;
+  bool bc_ready = false;
+  float* d_series = nullptr;
+  // run state
+  bool started = false;
+  bool run_set = false;
+  int npoints = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool graph_failed = false;
+  // stats / timing
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  fip_stats stats{};
+  std::string arch;
+};
+
+namespace {
+
+int launch_stencil(fip_sim* s, int par) {
+  const int blocks = s->masked ? s->stencil_blocks : (s->ntiles + 3) / 4;
+  hipError_t e;
+  if (s->timing) HIPTRY(hipEventRecord(s->ev0, s->stream));
+  e = s->fast ? fip_launch_stencil_fast(s->g, par, s->masked, blocks, s->stream)
+              : fip_launch_stencil_exact(s->g, par, s->masked, blocks, s->stream);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
+  if (s->timing) {
+    HIPTRY(hipEventRecord(s->ev1, s->stream));
+    HIPTRY(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPTRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->stats.stencil_ms += ms;
+    s->stats.timed_launches++;
+  }
+  s->stats.stencil_launches++;
+  return FIP_OK;
+}
+
+int launch_misc(fip_sim* s, int which, int arg, int blocks) {
+  hipError_t e = fip_launch_misc(which, s->g, arg, blocks, s->stream);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return FIP_OK;
+}
+
+// Boundary kernel modes.
+enum { BC_PRE0 = 0, BC_POST = 1, BC_APPLY_DEVICE = 2, BC_APPLY_HOST = 3 };
+
+int launch_bc(fip_sim* s, int mode, int par) {
+  if (!s->bc_ready) return fail(FIP_E_STATE, "boundary kernel not generated (call fip_bc_generate)");
+  int blocks = 1;
+  if (s->masked && (mode == BC_PRE0 || mode == BC_POST)) blocks += s->sched_blocks;
+  void* args[] = {&s->g, &mode, &par};
+  HIPTRY(hipModuleLaunchKernel(s->bc_fn, blocks, 1, 1, 1024, 1, 1, 0, s->stream, args, nullptr));
+  return FIP_OK;
+}
+
+template <class T>
+int ctl_write(fip_sim* s, size_t off, const T* v, size_t n = 1) {
+  HIPTRY(hipMemcpyAsync((char*)s->g.ctl + off, v, sizeof(T) * n, hipMemcpyHostToDevice, s->stream));
+  HIPTRY(hipStreamSynchronize(s->stream));
+  return FIP_OK;
+}
+
+template <class T>
+int ctl_read(fip_sim* s, size_t off, T* v, size_t n = 1) {
+  HIPTRY(hipMemcpyAsync(v, (char*)s->g.ctl + off, sizeof(T) * n, hipMemcpyDeviceToHost, s->stream));
+  HIPTRY(hipStreamSynchronize(s->stream));
+  return FIP_OK;
+}
+
+#define CTL_OFF(f) offsetof(fip_ctl, f)
+
+// Source of the boundary kernel for this BC list: the counterpart of the string templates
+// of src/bc_kernel_gen.cpp:11-152. One work-group applies the BCs in list order with a
+// barrier between them, so corner cells shared by two BCs get a defined result. Modes:
+//   PRE0  - before the first step: controlTimestep, sample, BC(t=0) (main.cpp:102-114)
+//   POST  - after a stencil step of parity par: duration += dt, iteration++, swap, BC(t),
+//           save_state / end-of-run checks, then controlTimestep and sample of the next
+//           simulate() call (main.cpp:116-135, 102-111)
+//   APPLY_DEVICE - BC(t) with parameters evaluated on the device
+//   APPLY_HOST   - BC with parameters uploaded by the host (BoundaryConditionKernel::launch)
+// Blocks other than 0 (masked mode, PRE0/POST) build the next active-tile list.
+std::string bc_source(const std::vector<BCDesc>& bcs, bool staggered, int W, int H) {
+  std::string s;
+  s += "#include \"fip_ctl.h\"\n#include \"fip_detmath.h\"\n#include \"fip_sched.h\"\n"
+       "#include \"fip_step_common.h\"\n\n";
+  const int ntid = (std::max(W, H) / 64 + 1) * 64;
+  char buf[1024];
+  for (size_t i = 0; i < bcs.size(); i++) {
+    const BCDesc& b = bcs[i];
+    if (b.type != FIP_BC_Q) continue;
+    // BoundaryCondition::sumWaterDepths (src/boundary_condition.cpp:6-18), cached per z.
+    const char* idx = b.side == FIP_SIDE_LEFT    ? "(long)n * g.P"
+                      : b.side == FIP_SIDE_RIGHT ? "(long)n * g.P + (g.W - 1)"
+                      : b.side == FIP_SIDE_TOP   ? "(long)(g.H - 1) * g.P + n"
+                                                 : "(long)n";
+    std::snprintf(buf, sizeof buf,
+                  "__device__ float fip_swd_%zu(const fip_grid& g, float z) {\n"
+                  "  fip_ctl* c = g.ctl;\n"
+                  "  if (c->swd_valid[%zu] && fip_f2u(c->swd_z[%zu]) == fip_f2u(z)) return c->swd_val[%zu];\n"
+                  "  float ah_sum = 0.f;\n"
+                  "  for (int n = %d; n < %d; n++) {\n"
+                  "    float ah = z - g.bed[%s];\n"
+                  "    if (ah < 0.01f) ah = 0.f;\n"
+                  "    ah_sum += __builtin_sqrtf(9.81f * ah) * ah;\n"
+                  "  }\n"
+                  "  ah_sum *= g.dx;\n"
+                  "  c->swd_z[%zu] = z; c->swd_val[%zu] = ah_sum; c->swd_valid[%zu] = 1;\n"
+                  "  return ah_sum;\n}\n\n",
+                  i, i, i, i, b.from, b.to, idx, i, i, i);
+    s += buf;
+  }
+  s += "extern \"C\" __global__ __launch_bounds__(1024) void fip_bc_kernel(fip_grid g, int mode, int par) {\n"
+       "  fip_ctl* c = g.ctl;\n"
+       "  if (blockIdx.x > 0) {\n"
+       "    if (mode == 0) { fip_schedule(g, 0, blockIdx.x - 1, gridDim.x - 1); return; }\n"
+       "    if (mode == 1 && !c->halt_req[par ^ 1]) fip_schedule(g, par ^ 1, blockIdx.x - 1, gridDim.x - 1);\n"
+       "    return;\n"
+       "  }\n";
+  std::snprintf(buf, sizeof buf,
+                "  __shared__ float s_q[%d], s_z[%d];\n  __shared__ int s_skip, s_cur;\n",
+                (int)std::max<size_t>(1, bcs.size()), (int)std::max<size_t>(1, bcs.size()));
+  s += buf;
+  s += "  if (threadIdx.x == 0) {\n"
+       "    int skip = 0, cur = par;\n"
+       "    if (mode == 1) {\n"
+       "      if (c->halt_req[par ^ 1]) { c->halt_req[par] = c->halt_req[par ^ 1]; skip = 1; }\n"
+       "      else {\n"
+       "        c->duration = c->duration + (double)c->dt;\n"
+       "        c->iteration = c->iteration + 1;\n"
+       "        c->cfl_prev = fip_u2f(c->cfl_acc[par]);\n"
+       "        c->cfl_acc[par] = fip_f2u(1.0f);\n"
+       "        c->list_count[par] = 0u;\n"
+       "        c->work_next[par] = 0u;\n"
+       "        cur = par ^ 1;\n"
+       "      }\n"
+       "    } else if (mode == 0) {\n"
+       "      if (c->variable_dt) fip_control(c);\n"
+       "      fip_sample(g, par);\n"
+       "    }\n"
+       "    if (!skip) {\n"
+       "      const float tf = (float)c->duration;\n"
+       "      (void)tf;\n";
+  for (size_t i = 0; i < bcs.size(); i++) {
+    const BCDesc& b = bcs[i];
+    std::string zexpr, qexpr;
+    const bool series = b.zn > 0 || b.qn > 0;
+    if (!series) {
+      zexpr = hexf(b.z);
+      qexpr = b.type == FIP_BC_Q ? "(" + hexf(b.q) + " / fip_swd_" + std::to_string(i) + "(g, zv))"
+                                 : hexf(b.q);
+    } else {
+      // getZ/getQ of BoundaryConditionTimeseries (src/boundary_condition.cpp:66-77); a
+      // series BC without a z series reads z as 0.
+      if (b.zn > 0) {
+        std::snprintf(buf, sizeof buf, "fip_interp(g.series + %d, g.series + %d, %d, tf)", b.zoff,
+                      b.zoff + b.zn, b.zn);
+        zexpr = buf;
+      } else {
+        zexpr = "0.f";
+      }
+      if (b.type == FIP_BC_Q) {
+        std::string qs = "0.f";
+        if (b.qn > 0) {
+          std::snprintf(buf, sizeof buf, "fip_interp(g.series + %d, g.series + %d, %d, tf)",
+                        b.qoff, b.qoff + b.qn, b.qn);
+          qs = buf;
+        }
+        qexpr = "(" + qs + " / fip_swd_" + std::to_string(i) + "(g, zv))";
+      } else {
+        qexpr = "0.f";
+      }
+    }
+    std::snprintf(buf, sizeof buf,
+                  "      if (mode == 3) { s_q[%zu] = c->bc_q[%zu]; s_z[%zu] = c->bc_z[%zu]; }\n"
+                  "      else { const float zv = %s; const float qv = %s;\n"
+                  "             s_q[%zu] = qv; s_z[%zu] = zv; c->bc_q[%zu] = qv; c->bc_z[%zu] = zv; }\n",
+                  i, i, i, i, zexpr.c_str(), qexpr.c_str(), i, i, i, i);
+    s += buf;
+  }
+  s += "    }\n    s_skip = skip;\n    s_cur = cur;\n  }\n  __syncthreads();\n"
+       "  if (s_skip) return;\n"
+       "  const int cur = s_cur;\n"
+       "  float* h = g.h[cur];\n  float* qx = g.qx[cur];\n  float* qy = g.qy[cur];\n"
+       "  const float* sohle = g.bed;\n  (void)h; (void)qx; (void)qy; (void)sohle;\n";
+  for (size_t i = 0; i < bcs.size(); i++) {
+    const BCDesc& b = bcs[i];
+    const char *idx, *stag, *nb, *q, *ql;
+    if (b.side == FIP_SIDE_RIGHT) {
+      idx = "(long)tid * g.P + (g.W - 1)"; stag = "idx - 1"; nb = "idx - 1"; q = "qx"; ql = "qy";
+    } else if (b.side == FIP_SIDE_LEFT) {
+      idx = "(long)tid * g.P"; stag = "idx"; nb = "idx + 1"; q = "qx"; ql = "qy";
+    } else if (b.side == FIP_SIDE_TOP) {
+      idx = "(long)tid + (long)(g.H - 1) * g.P"; stag = "idx - g.P"; nb = "idx - g.P"; q = "qy"; ql = "qx";
+    } else {
+      idx = "(long)tid"; stag = "idx"; nb = "idx + g.P"; q = "qy"; ql = "qx";
+    }
+    // direction of the open-BC discharge: +1 on the right/top side, -1 on the left/bottom
+    const char* sign = (b.side == FIP_SIDE_RIGHT || b.side == FIP_SIDE_TOP) ? "1.0f" : "(-1.0f)";
+    std::string body;
+    const std::string Q = std::string(q), QL = std::string(ql);
+    const std::string qi = staggered ? "stag" : "idx";
+    if (b.type == FIP_BC_Z) {
+      if (staggered)
+        body = "h[idx] = fip_fmaxf(0.f, zval - sohle[idx]);\n"
+               "      h[nb] = fip_fmaxf(0.f, zval - sohle[nb]);\n"
+               "      " + Q + "[stag] = -0.f;\n";
+      else
+        body = "float ah = fip_fmaxf(0.f, zval - sohle[idx]);\n"
+               "      h[idx] = ah;\n"
+               "      " + Q + "[idx] = 0.99f * (" + Q + "[nb] / fip_fmaxf(h[nb], 0.01f)) * ah;\n";
+    } else if (b.type == FIP_BC_Q) {
+      if (staggered)
+        body = "float ah = zval - sohle[idx];\n      if (ah < 0.f) ah = 0.f;\n"
+               "      " + Q + "[stag] = __builtin_sqrtf(9.81f * ah) * ah * qval;\n"
+               "      if (ah > 0.f) { float z_neighbor = h[nb] + sohle[nb];\n"
+               "        h[idx] = fip_fmaxf(ah / 2, z_neighbor - sohle[idx]); }\n";
+      else
+        body = "float ah = zval - sohle[idx];\n      if (ah < 0.f) ah = 0.f;\n"
+               "      " + Q + "[idx] = __builtin_sqrtf(9.81f * ah) * ah * qval;\n"
+               "      if (ah > 0.f) h[idx] = fip_fmaxf(ah / 2, h[nb] + sohle[nb] - sohle[idx]);\n";
+    } else if (b.type == FIP_BC_CLOSE) {
+      body = Q + "[" + qi + "] = 0.f;\n      " + QL + "[idx] = " + QL + "[nb];\n      h[idx] = h[nb];\n";
+    } else {
+      body = "float ah_neighbor = h[nb];\n      float z_neighbor = ah_neighbor + sohle[nb];\n"
+             "      if (ah_neighbor > 0.f) {\n"
+             "        " + Q + "[" + qi + "] = " + sign + " * __builtin_sqrtf(9.81f * ah_neighbor) * ah_neighbor;\n"
+             "        h[idx] = fip_fminf(ah_neighbor, fip_fmaxf(0.f, z_neighbor - sohle[idx])); }\n";
+    }
+    const int lim = std::min(b.to, b.from + ntid - 1);
+    std::snprintf(buf, sizeof buf,
+                  "  { const float qval = s_q[%zu], zval = s_z[%zu]; (void)qval; (void)zval;\n"
+                  "    for (int tid = %d + (int)threadIdx.x; tid <= %d; tid += blockDim.x) {\n"
+                  "      const long idx = %s; const long stag = %s; const long nb = %s;\n"
+                  "      (void)stag;\n      ",
+                  i, i, b.from, lim, idx, stag, nb);
+    s += buf;
+    s += body;
+    s += "    }\n  }\n  __syncthreads();\n";
+  }
+  s += "  if (mode == 1 && threadIdx.x == 0) {\n"
+       "    int req = 0;\n"
+       "    if (c->save_pos < c->nsave && c->duration >= c->save_times[c->save_pos]) {\n"
+       "      c->save_pos = c->save_pos + 1; req |= FIP_HALT_DUMP; }\n"
+       "    if (!(c->duration < c->target)) req |= FIP_HALT_FINISHED;\n"
+       "    c->halt_req[par] = req;\n"
+       "    if (!(req & FIP_HALT_FINISHED)) {\n"
+       "      if (c->variable_dt) fip_control(c);\n"
+       "      fip_sample(g, cur);\n"
+       "    }\n"
+       "  }\n"
+       "}\n";
+  return s;
+}
+
+int compile_bc(fip_sim* s, const std::string& src) {
+  hiprtcProgram prog;
+  std::vector<const char*> hdr_src, hdr_names;
+  for (int i = 0; i < kEmbedCount; i++) {
+    hdr_src.push_back(kEmbedSrc[i]);
+    hdr_names.push_back(kEmbedNames[i]);
+  }
+  if (hiprtcCreateProgram(&prog, src.c_str(), "fip_bc_kernel.hip", kEmbedCount, hdr_src.data(),
+                          hdr_names.data()) != HIPRTC_SUCCESS)
+    return fail(FIP_E_RTC, "hiprtcCreateProgram failed");
+  std::string archopt = "--offload-arch=" + s->arch;
+  const char* opts[] = {archopt.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    return fail(FIP_E_RTC, "hipRTC compile of the boundary kernel failed:\n" + log);
+  }
+  size_t code_size = 0;
+  hiprtcGetCodeSize(prog, &code_size);
+  std::vector<char> code(code_size);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  if (s->module) {
+    hipModuleUnload(s->module);
+    s->module = nullptr;
+  }
+  HIPTRY(hipModuleLoadData(&s->module, code.data()));
+  HIPTRY(hipModuleGetFunction(&s->bc_fn, s->module, "fip_bc_kernel"));
+  return FIP_OK;
+}
+
+void destroy_graphs(fip_sim* s) {
+  for (auto& gx : s->graph)
+    if (gx) {
+      hipGraphExecDestroy(gx);
+      gx = nullptr;
+    }
+}
+
+int one_step(fip_sim* s, int par) {
+  int rc = launch_stencil(s, par);
+  if (rc) return rc;
+  return launch_bc(s, BC_POST, par);
+}
+
+// Captures kStepsPerGraph steps starting at parity par, plus the flood-plain kernel.
+int build_graph(fip_sim* s, int par) {
+  hipGraph_t graph;
+  HIPTRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = FIP_OK;
+  for (int k = 0; k < kStepsPerGraph && rc == FIP_OK; k++) rc = one_step(s, par ^ (k & 1));
+  if (rc == FIP_OK) rc = launch_misc(s, 3, 1, 0);
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (rc != FIP_OK) return rc;
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&s->graph[par], graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return FIP_OK;
+}
+
+int validate_bc(const fip_desc& d, const fip_bc& b) {
+  if (b.type < 0 || b.type > 3) return fail(FIP_E_INVALID, "invalid boundary condition type");
+  if (b.side < 0 || b.side > 3) return fail(FIP_E_INVALID, "invalid boundary condition side");
+  const int ext = (b.side == FIP_SIDE_LEFT || b.side == FIP_SIDE_RIGHT) ? d.H : d.W;
+  if (b.from < 0 || b.to >= ext)
+    return fail(FIP_E_INVALID, "boundary condition range [" + std::to_string(b.from) + ", " +
+                                   std::to_string(b.to) + "] outside the side's " +
+                                   std::to_string(ext) + " cells");
+  for (int k = 1; k < b.zn; k++)
+    if (!(b.zt[k] > b.zt[k - 1])) return fail(FIP_E_INVALID, "z series times must increase");
+  for (int k = 1; k < b.qn; k++)
+    if (!(b.qt[k] > b.qt[k - 1])) return fail(FIP_E_INVALID, "q series times must increase");
+  return FIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fip_last_error(void) { return g_err.c_str(); }
+
+int fip_create(const fip_desc* d, fip_sim** out) {
+  if (!d || !out) return fail(FIP_E_INVALID, "null argument");
+  *out = nullptr;
+  if (d->W < 4 || d->H < 4) return fail(FIP_E_INVALID, "W and H must be at least 4");
+  if (!d->bed) return fail(FIP_E_INVALID, "bed (terrain) is required");
+  if (!(d->dx > 0.f)) return fail(FIP_E_INVALID, "dx must be positive");
+  if (!(d->dt > 0.f)) return fail(FIP_E_INVALID, "dt must be positive");
+  const int R = d->tile_rows == 0 ? 64 : d->tile_rows;
+  if (R != 16 && R != 32 && R != 64) return fail(FIP_E_INVALID, "tile_rows must be 16, 32 or 64");
+  if (d->mode != FIP_MODE_DENSE && d->mode != FIP_MODE_MASKED)
+    return fail(FIP_E_INVALID, "invalid mode");
+  if (d->variant != FIP_EXACT && d->variant != FIP_FAST) return fail(FIP_E_INVALID, "invalid variant");
+
+  fip_sim* s = new fip_sim();
+  s->d = *d;
+  s->d.tile_rows = R;
+  s->dev = d->device;
+  s->masked = d->mode == FIP_MODE_MASKED;
+  s->fast = d->variant == FIP_FAST;
+  auto cleanup = [&](int rc) {
+    fip_destroy(s);
+    return rc;
+  };
+  if (hipSetDevice(s->dev) != hipSuccess) return cleanup(fail(FIP_E_HIP, "hipSetDevice failed (no GPU?)"));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, s->dev) != hipSuccess)
+    return cleanup(fail(FIP_E_HIP, "hipGetDeviceProperties failed"));
+  s->arch = prop.gcnArchName;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(FIP_E_HIP, "hipStreamCreate failed"));
+
+  const int W = d->W, H = d->H;
+  const int P = ((kMarginCols + W + kMarginCols) + 63) / 64 * 64;
+  const int rows = H + 2 * kMarginRows;
+  s->field_elems = (size_t)P * rows;
+  s->interior_off = (long)kMarginRows * P + kMarginCols;
+  fip_grid& g = s->g;
+  g.W = W;
+  g.H = H;
+  g.P = P;
+  g.tile_rows = R;
+  g.tiles_x = (W + kTileCols - 1) / kTileCols;
+  g.tiles_y = (H + R - 1) / R;
+  s->ntiles = g.tiles_x * g.tiles_y;
+  g.dx = d->dx;
+  g.no_data = d->no_data;
+  g.kst_uniform = d->kst;
+  const int nfield = d->kst_var ? 9 : 8;
+  for (int k = 0; k < nfield; k++) {
+    if (hipMalloc(&s->base_alloc[k], s->field_elems * sizeof(float)) != hipSuccess)
+      return cleanup(fail(FIP_E_HIP, "hipMalloc of a field failed"));
+    if (hipMemset(s->base_alloc[k], 0, s->field_elems * sizeof(float)) != hipSuccess)
+      return cleanup(fail(FIP_E_HIP, "hipMemset failed"));
+  }
+  float* in[9];
+  for (int k = 0; k < nfield; k++) in[k] = s->base_alloc[k] + s->interior_off;
+  g.h[0] = in[0]; g.h[1] = in[1];
+  g.qx[0] = in[2]; g.qx[1] = in[3];
+  g.qy[0] = in[4]; g.qy[1] = in[5];
+  g.hmax = in[6];
+  g.bed = in[7];
+  g.kst = d->kst_var ? in[8] : nullptr;
+  // initial state into both ping-pong sides (src/simdata.cpp:82-90), hmax starts as h
+  const size_t pitch = (size_t)P * sizeof(float), wb = (size_t)W * sizeof(float);
+  std::vector<float> zeros;
+  auto up = [&](float* dst, const float* src) -> int {
+    if (!src) return FIP_OK;
+    if (hipMemcpy2D(dst, pitch, src, wb, wb, H, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(FIP_E_HIP, "hipMemcpy2D upload failed");
+    return FIP_OK;
+  };
+  int rc = FIP_OK;
+  rc |= up(g.h[0], d->h); rc |= up(g.h[1], d->h); rc |= up(g.hmax, d->h);
+  rc |= up(g.qx[0], d->qx); rc |= up(g.qx[1], d->qx);
+  rc |= up(g.qy[0], d->qy); rc |= up(g.qy[1], d->qy);
+  rc |= up((float*)g.bed, d->bed);
+  if (d->kst_var) rc |= up((float*)g.kst, d->kst_var);
+  if (rc) return cleanup(FIP_E_HIP);
+  // masks: every tile wet in every row (src/simdata.cpp:77-79)
+  std::vector<unsigned long long> masks(s->ntiles);
+  for (int t = 0; t < s->ntiles; t++) {
+    const int ty = t / g.tiles_x;
+    const int nr = std::min(R, H - ty * R);
+    masks[t] = nr >= 64 ? ~0ull : ((1ull << nr) - 1ull);
+  }
+  if (hipMalloc(&g.mask, sizeof(unsigned long long) * s->ntiles) != hipSuccess ||
+      hipMemcpy(g.mask, masks.data(), sizeof(unsigned long long) * s->ntiles, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(FIP_E_HIP, "mask allocation failed"));
+  for (int k = 0; k < 2; k++)
+    if (hipMalloc(&g.list[k], sizeof(unsigned) * s->ntiles) != hipSuccess)
+      return cleanup(fail(FIP_E_HIP, "tile list allocation failed"));
+  fip_ctl* hc = new fip_ctl();
+  std::memset(hc, 0, sizeof(fip_ctl));
+  hc->dt = d->dt;
+  hc->cfl_prev = 1.0f;
+  const float one = 1.0f;
+  std::memcpy(&hc->cfl_acc[0], &one, 4);
+  std::memcpy(&hc->cfl_acc[1], &one, 4);
+  hc->variable_dt = d->variable_dt;
+  hc->target = 0.0;
+  bool ok = hipMalloc(&g.ctl, sizeof(fip_ctl)) == hipSuccess &&
+            hipMemcpy(g.ctl, hc, sizeof(fip_ctl), hipMemcpyHostToDevice) == hipSuccess;
+  delete hc;
+  if (!ok) return cleanup(fail(FIP_E_HIP, "control block allocation failed"));
+  int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  s->stencil_blocks = std::max(1, std::min((s->ntiles + 3) / 4, cus * 8));
+  s->sched_blocks = std::max(1, std::min((s->ntiles + 255) / 256, cus));
+  if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+    return cleanup(fail(FIP_E_HIP, "hipEventCreate failed"));
+  s->stats.tiles_total = s->ntiles;
+  if (hipDeviceSynchronize() != hipSuccess) return cleanup(fail(FIP_E_HIP, "device sync failed"));
+  *out = s;
+  return FIP_OK;
+}
+
+int fip_destroy(fip_sim* s) {
+  if (!s) return FIP_OK;
+  hipSetDevice(s->dev);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  destroy_graphs(s);
+  for (auto& p : s->base_alloc)
+    if (p) hipFree(p);
+  if (s->g.mask) hipFree(s->g.mask);
+  for (auto& l : s->g.list)
+    if (l) hipFree(l);
+  if (s->g.ctl) hipFree(s->g.ctl);
+  if (s->d_series) hipFree(s->d_series);
+  if (s->module) hipModuleUnload(s->module);
+  if (s->ev0) hipEventDestroy(s->ev0);
+  if (s->ev1) hipEventDestroy(s->ev1);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return FIP_OK;
+}
+
+static int collect_bcs(const fip_desc& d, const fip_bc* bcs, int n, std::vector<BCDesc>& v,
+                       std::vector<float>& table) {
+  if (n > 0 && !bcs) return fail(FIP_E_INVALID, "null argument");
+  if (n < 0 || n > FIP_MAX_BC) return fail(FIP_E_INVALID, "too many boundary conditions");
+  for (int i = 0; i < n; i++) {
+    int rc = validate_bc(d, bcs[i]);
+    if (rc) return rc;
+    BCDesc b{bcs[i].type, bcs[i].side, bcs[i].from, bcs[i].to, bcs[i].z, bcs[i].q, 0, 0, 0, 0};
+    if (bcs[i].zn > 0) {
+      b.zoff = (int)table.size();
+      b.zn = bcs[i].zn;
+      table.insert(table.end(), bcs[i].zt, bcs[i].zt + b.zn);
+      table.insert(table.end(), bcs[i].zv, bcs[i].zv + b.zn);
+    }
+    if (bcs[i].qn > 0) {
+      b.qoff = (int)table.size();
+      b.qn = bcs[i].qn;
+      table.insert(table.end(), bcs[i].qt, bcs[i].qt + b.qn);
+      table.insert(table.end(), bcs[i].qv, bcs[i].qv + b.qn);
+    }
+    v.push_back(b);
+  }
+  return FIP_OK;
+}
+
+int fip_bc_source(const fip_bc* bcs, int n, int W, int H, int staggered, char* buf, size_t cap,
+                  size_t* len) {
+  fip_desc d{};
+  d.W = W;
+  d.H = H;
+  std::vector<BCDesc> v;
+  std::vector<float> table;
+  int rc = collect_bcs(d, bcs, n, v, table);
+  if (rc) return rc;
+  const std::string src = bc_source(v, staggered != 0, W, H);
+  if (len) *len = src.size();
+  if (buf && cap > 0) {
+    const size_t k = std::min(cap - 1, src.size());
+    std::memcpy(buf, src.data(), k);
+    buf[k] = '\0';
+  }
+  return FIP_OK;
+}
+
+int fip_bc_generate(fip_sim* s, const fip_bc* bcs, int n) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  std::vector<BCDesc> v;
+  std::vector<float> table;
+  int rc = collect_bcs(s->d, bcs, n, v, table);
+  if (rc) return rc;
+  if (s->d_series) {
+    hipFree(s->d_series);
+    s->d_series = nullptr;
+  }
+  if (!table.empty()) {
+    HIPTRY(hipMalloc(&s->d_series, table.size() * sizeof(float)));
+    HIPTRY(hipMemcpy(s->d_series, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  s->g.series = s->d_series;
+  rc = compile_bc(s, bc_source(v, s->d.staggered != 0, s->d.W, s->d.H));
+  if (rc) return rc;
+  // reset the sumWaterDepths cache
+  int zeros[FIP_MAX_BC] = {0};
+  rc = ctl_write(s, CTL_OFF(swd_valid), zeros, FIP_MAX_BC);
+  if (rc) return rc;
+  s->nbc = n;
+  s->bc_ready = true;
+  destroy_graphs(s);
+  return FIP_OK;
+}
+
+int fip_bc_launch(fip_sim* s, const float* params) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  if (s->nbc > 0) {
+    std::vector<float> q(s->nbc), z(s->nbc);
+    for (int i = 0; i < s->nbc; i++) {
+      q[i] = params[2 * i];
+      z[i] = params[2 * i + 1];
+    }
+    int rc = ctl_write(s, CTL_OFF(bc_q), q.data(), s->nbc);
+    if (!rc) rc = ctl_write(s, CTL_OFF(bc_z), z.data(), s->nbc);
+    if (rc) return rc;
+  }
+  return launch_bc(s, BC_APPLY_HOST, s->host_iter & 1);
+}
+
+int fip_bc_apply(fip_sim* s) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  return launch_bc(s, BC_APPLY_DEVICE, s->host_iter & 1);
+}
+
+int fip_bc_params(fip_sim* s, float* params, int n) {
+  if (!s || !params) return fail(FIP_E_INVALID, "null argument");
+  n = std::min(n, s->nbc);
+  std::vector<float> q(n), z(n);
+  int rc = ctl_read(s, CTL_OFF(bc_q), q.data(), n);
+  if (!rc) rc = ctl_read(s, CTL_OFF(bc_z), z.data(), n);
+  for (int i = 0; i < n; i++) {
+    params[2 * i] = q[i];
+    params[2 * i + 1] = z[i];
+  }
+  return rc;
+}
+
+int fip_control_timestep(fip_sim* s) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  return launch_misc(s, 0, 0, 1);
+}
+
+int fip_compute_step(fip_sim* s) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  const int par = s->host_iter & 1;
+  if (s->masked && !s->started) {
+    // first step of a fine-grained run: build the initial list from the all-wet masks
+    unsigned zero = 0;
+    int rc = ctl_write(s, CTL_OFF(list_count[0]), &zero);
+    if (!rc) rc = ctl_write(s, CTL_OFF(list_count[1]), &zero);
+    if (!rc) rc = launch_misc(s, 2, par, s->sched_blocks);
+    if (rc) return rc;
+    s->started = true;
+  }
+  int rc = launch_stencil(s, par);
+  if (!rc) rc = launch_misc(s, 1, par, 1);
+  if (!rc && s->masked) rc = launch_misc(s, 2, par ^ 1, s->sched_blocks);
+  if (rc) return rc;
+  s->host_iter++;
+  s->stats.steps++;
+  return FIP_OK;
+}
+
+int fip_floodplain(fip_sim* s) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  return launch_misc(s, 3, 0, 0);
+}
+
+int fip_set_run(fip_sim* s, double target, int sampling_interval, const int* sx, const int* sy,
+                const int* st, int npoints, const double* save_times, int nsave) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  if (npoints < 0 || npoints > FIP_MAX_SAMPLE_POINTS)
+    return fail(FIP_E_INVALID, "too many sampling points (max " + std::to_string(FIP_MAX_SAMPLE_POINTS) + ")");
+  if (nsave < 0 || nsave > FIP_MAX_SAVE)
+    return fail(FIP_E_INVALID, "too many save_state times (max " + std::to_string(FIP_MAX_SAVE) + ")");
+  for (int p = 0; p < npoints; p++)
+    if (sx[p] < 0 || sx[p] >= s->d.W || sy[p] < 0 || sy[p] >= s->d.H || st[p] < 0 || st[p] > 3)
+      return fail(FIP_E_INVALID, "sampling point outside the grid");
+  HIPTRY(hipSetDevice(s->dev));
+  fip_ctl* hc = new fip_ctl();
+  int rc = ctl_read(s, 0, hc);
+  if (rc) {
+    delete hc;
+    return rc;
+  }
+  hc->target = target;
+  hc->sampling_interval = sampling_interval;
+  hc->sample_timer = 0.0;  // Sampling::timer{duration} (src/simdata.cpp:41)
+  hc->nsample_points = npoints;
+  for (int p = 0; p < npoints; p++) {
+    hc->sp_x[p] = sx[p];
+    hc->sp_y[p] = sy[p];
+    hc->sp_type[p] = st[p];
+  }
+  std::vector<double> sv(save_times, save_times + nsave);
+  std::sort(sv.begin(), sv.end());
+  for (int k = 0; k < nsave; k++) hc->save_times[k] = sv[k];
+  hc->nsave = nsave;
+  hc->save_pos = 0;
+  rc = ctl_write(s, 0, hc);
+  delete hc;
+  s->npoints = npoints;
+  s->run_set = true;
+  return rc;
+}
+
+int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done) {
+  if (!s || !event) return fail(FIP_E_INVALID, "null argument");
+  *event = FIP_EV_NONE;
+  if (steps_done) *steps_done = 0;
+  if (!s->run_set) return fail(FIP_E_STATE, "call fip_set_run first");
+  HIPTRY(hipSetDevice(s->dev));
+  int rc;
+  // pending stop requests: a finished run stays finished; a handled dump is cleared
+  int req[2];
+  if ((rc = ctl_read(s, CTL_OFF(halt_req), req, 2))) return rc;
+  if ((req[0] | req[1]) & FIP_HALT_FINISHED) {
+    *event = FIP_EV_FINISHED;
+    return FIP_OK;
+  }
+  if (req[0] | req[1]) {
+    int z[2] = {0, 0};
+    if ((rc = ctl_write(s, CTL_OFF(halt_req), z, 2))) return rc;
+  }
+  if (!s->started) {
+    double t, target;
+    if ((rc = ctl_read(s, CTL_OFF(duration), &t))) return rc;
+    if ((rc = ctl_read(s, CTL_OFF(target), &target))) return rc;
+    if (!(t < target)) {
+      *event = FIP_EV_FINISHED;
+      return FIP_OK;
+    }
+    if ((rc = launch_bc(s, BC_PRE0, s->host_iter & 1))) return rc;
+    s->started = true;
+  }
+  long long done = 0;
+  while (done < nsteps) {
+    const int it0 = s->host_iter;
+    const long long to_flood = kStepsPerGraph - (it0 % kStepsPerGraph);
+    const long long k = std::min<long long>(to_flood, nsteps - done);
+    const bool use_graph = k == kStepsPerGraph && !s->timing && !s->graph_failed;
+    if (use_graph && !s->graph[it0 & 1]) {
+      if (build_graph(s, it0 & 1) != FIP_OK) {
+        s->graph_failed = true;
+        hipGetLastError();
+      }
+    }
+    if (use_graph && s->graph[it0 & 1]) {
+      HIPTRY(hipGraphLaunch(s->graph[it0 & 1], s->stream));
+      s->stats.stencil_launches += k;
+    } else {
+      for (long long i = 0; i < k; i++)
+        if ((rc = one_step(s, (it0 + (int)i) & 1))) return rc;
+      if ((rc = launch_misc(s, 3, 1, 0))) return rc;
+    }
+    int it1;
+    if ((rc = ctl_read(s, CTL_OFF(iteration), &it1))) return rc;
+    if ((rc = ctl_read(s, CTL_OFF(halt_req), req, 2))) return rc;
+    s->stats.steps += it1 - it0;
+    done += it1 - it0;
+    s->host_iter = it1;
+    const int r = req[0] | req[1];
+    if (r & FIP_HALT_DUMP) {
+      *event = FIP_EV_DUMP;
+      break;
+    }
+    if (r & FIP_HALT_FINISHED) {
+      *event = FIP_EV_FINISHED;
+      break;
+    }
+  }
+  if (steps_done) *steps_done = done;
+  if (s->masked) {
+    unsigned cnt[2];
+    if ((rc = ctl_read(s, CTL_OFF(list_count), cnt, 2))) return rc;
+    s->stats.tiles_computed = cnt[s->host_iter & 1];
+  } else {
+    s->stats.tiles_computed = s->ntiles;
+  }
+  return FIP_OK;
+}
+
+int fip_read_samples(fip_sim* s, double* t, float* vals, int max_rows, int* nrows) {
+  if (!s || !nrows) return fail(FIP_E_INVALID, "null argument");
+  int n = 0, ovf = 0;
+  int rc = ctl_read(s, CTL_OFF(nsample_rows), &n);
+  if (!rc) rc = ctl_read(s, CTL_OFF(sample_overflow), &ovf);
+  if (rc) return rc;
+  if (ovf) return fail(FIP_E_STATE, "sample log overflow");
+  n = std::min(n, max_rows);
+  if (n > 0) {
+    std::vector<double> tt(n);
+    std::vector<float> vv((size_t)n * FIP_MAX_SAMPLE_POINTS);
+    if ((rc = ctl_read(s, CTL_OFF(sample_t), tt.data(), n))) return rc;
+    if ((rc = ctl_read(s, CTL_OFF(sample_v), vv.data(), vv.size()))) return rc;
+    for (int r = 0; r < n; r++) {
+      t[r] = tt[r];
+      for (int p = 0; p < s->npoints; p++)
+        vals[(size_t)r * s->npoints + p] = vv[(size_t)r * FIP_MAX_SAMPLE_POINTS + p];
+    }
+    // drain: move rows beyond n (none when max_rows is large enough) and reset
+    int zero = 0;
+    if ((rc = ctl_write(s, CTL_OFF(nsample_rows), &zero))) return rc;
+  }
+  *nrows = n;
+  return FIP_OK;
+}
+
+int fip_get_time(fip_sim* s, double* duration, float* dt, int* iteration) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  int rc;
+  if (duration && (rc = ctl_read(s, CTL_OFF(duration), duration))) return rc;
+  if (dt && (rc = ctl_read(s, CTL_OFF(dt), dt))) return rc;
+  if (iteration && (rc = ctl_read(s, CTL_OFF(iteration), iteration))) return rc;
+  return FIP_OK;
+}
+
+int fip_set_dt(fip_sim* s, float dt) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  return ctl_write(s, CTL_OFF(dt), &dt);
+}
+
+static float* field_ptr(fip_sim* s, int field, int side) {
+  const int k = (s->host_iter & 1) ^ (side ? 1 : 0);
+  switch (field) {
+    case FIP_FIELD_H: return s->g.h[k];
+    case FIP_FIELD_QX: return s->g.qx[k];
+    case FIP_FIELD_QY: return s->g.qy[k];
+    case FIP_FIELD_HMAX: return s->g.hmax;
+    case FIP_FIELD_BED: return (float*)s->g.bed;
+    case FIP_FIELD_KST: return (float*)s->g.kst;
+    default: return nullptr;
+  }
+}
+
+int fip_download(fip_sim* s, int field, int side, float* dst) {
+  if (!s || !dst) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  HIPTRY(hipStreamSynchronize(s->stream));
+  float* src = field_ptr(s, field, side);
+  if (!src) return fail(FIP_E_INVALID, "invalid field");
+  const size_t wb = (size_t)s->d.W * sizeof(float);
+  HIPTRY(hipMemcpy2D(dst, wb, src, (size_t)s->g.P * sizeof(float), wb, s->d.H, hipMemcpyDeviceToHost));
+  return FIP_OK;
+}
+
+int fip_upload(fip_sim* s, int field, int side, const float* src) {
+  if (!s || !src) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  HIPTRY(hipStreamSynchronize(s->stream));
+  float* dst = field_ptr(s, field, side);
+  if (!dst) return fail(FIP_E_INVALID, "invalid field");
+  const size_t wb = (size_t)s->d.W * sizeof(float);
+  HIPTRY(hipMemcpy2D(dst, (size_t)s->g.P * sizeof(float), src, wb, wb, s->d.H, hipMemcpyHostToDevice));
+  return FIP_OK;
+}
+
+int fip_tiles(fip_sim* s, int* tx, int* ty, int* tc, int* tr) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  if (tx) *tx = s->g.tiles_x;
+  if (ty) *ty = s->g.tiles_y;
+  if (tc) *tc = kTileCols;
+  if (tr) *tr = s->g.tile_rows;
+  return FIP_OK;
+}
+
+int fip_get_masks(fip_sim* s, uint64_t* dst, int n) {
+  if (!s || !dst) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  HIPTRY(hipStreamSynchronize(s->stream));
+  n = std::min(n, s->ntiles);
+  HIPTRY(hipMemcpy(dst, s->g.mask, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+  return FIP_OK;
+}
+
+int fip_get_cfl(fip_sim* s, float* cfl) {
+  if (!s || !cfl) return fail(FIP_E_INVALID, "null argument");
+  return ctl_read(s, CTL_OFF(cfl_prev), cfl);
+}
+
+int fip_set_timing(fip_sim* s, int enable) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  s->timing = enable != 0;
+  return FIP_OK;
+}
+
+int fip_get_stats(fip_sim* s, fip_stats* out) {
+  if (!s || !out) return fail(FIP_E_INVALID, "null argument");
+  *out = s->stats;
+  return FIP_OK;
+}
+
+int fip_sync(fip_sim* s) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  HIPTRY(hipSetDevice(s->dev));
+  HIPTRY(hipStreamSynchronize(s->stream));
+  return FIP_OK;
+}
+
+}  // extern "C"

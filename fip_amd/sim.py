@@ -1,0 +1,259 @@
+"""ctypes binding of libfiphip (include/fip.h). No fallback: if the HIP library is
+missing or a call fails, this raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .workloads import BC_SIDES, BC_TYPES, Case
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfiphip.so")
+
+FIELDS = {"h": 0, "qx": 1, "qy": 2, "hmax": 3, "bed": 4, "kst": 5}
+EV_NONE, EV_FINISHED, EV_DUMP = 0, 1, 2
+EXPORTS = ["fip_last_error", "fip_create", "fip_destroy", "fip_bc_generate", "fip_bc_launch",
+           "fip_bc_apply", "fip_bc_params", "fip_control_timestep", "fip_compute_step",
+           "fip_floodplain", "fip_set_run", "fip_simulate", "fip_read_samples", "fip_get_time",
+           "fip_set_dt", "fip_download", "fip_upload", "fip_tiles", "fip_get_masks",
+           "fip_get_cfl", "fip_set_timing", "fip_get_stats", "fip_sync"]
+
+
+class FipDesc(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("dx", C.c_float), ("dt", C.c_float),
+                ("no_data", C.c_float), ("kst", C.c_float), ("bed", C.c_void_p),
+                ("h", C.c_void_p), ("qx", C.c_void_p), ("qy", C.c_void_p),
+                ("kst_var", C.c_void_p), ("variable_dt", C.c_int), ("staggered", C.c_int),
+                ("mode", C.c_int), ("variant", C.c_int), ("tile_rows", C.c_int),
+                ("device", C.c_int)]
+
+
+class FipBC(C.Structure):
+    _fields_ = [("type", C.c_int), ("side", C.c_int), ("from_", C.c_int), ("to", C.c_int),
+                ("z", C.c_float), ("q", C.c_float), ("zt", C.c_void_p), ("zv", C.c_void_p),
+                ("zn", C.c_int), ("qt", C.c_void_p), ("qv", C.c_void_p), ("qn", C.c_int)]
+
+
+class FipStats(C.Structure):
+    _fields_ = [("steps", C.c_longlong), ("tiles_total", C.c_longlong),
+                ("tiles_computed", C.c_longlong), ("stencil_launches", C.c_longlong),
+                ("stencil_ms", C.c_double), ("timed_launches", C.c_longlong)]
+
+
+class FipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FipError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.fip_last_error.restype = C.c_char_p
+        vp = C.c_void_p
+        L.fip_create.argtypes = [C.POINTER(FipDesc), C.POINTER(vp)]
+        L.fip_destroy.argtypes = [vp]
+        L.fip_bc_generate.argtypes = [vp, C.POINTER(FipBC), C.c_int]
+        L.fip_bc_launch.argtypes = [vp, C.POINTER(C.c_float)]
+        L.fip_bc_apply.argtypes = [vp]
+        L.fip_bc_params.argtypes = [vp, C.POINTER(C.c_float), C.c_int]
+        for nm in ("fip_control_timestep", "fip_compute_step", "fip_floodplain", "fip_sync"):
+            getattr(L, nm).argtypes = [vp]
+        ip = C.POINTER(C.c_int)
+        L.fip_set_run.argtypes = [vp, C.c_double, C.c_int, ip, ip, ip, C.c_int,
+                                  C.POINTER(C.c_double), C.c_int]
+        L.fip_simulate.argtypes = [vp, C.c_longlong, ip, C.POINTER(C.c_longlong)]
+        L.fip_read_samples.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_float), C.c_int, ip]
+        L.fip_get_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_float), ip]
+        L.fip_set_dt.argtypes = [vp, C.c_float]
+        L.fip_download.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        L.fip_upload.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        L.fip_tiles.argtypes = [vp, ip, ip, ip, ip]
+        L.fip_get_masks.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int]
+        L.fip_get_cfl.argtypes = [vp, C.POINTER(C.c_float)]
+        L.fip_set_timing.argtypes = [vp, C.c_int]
+        L.fip_get_stats.argtypes = [vp, C.POINTER(FipStats)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise FipError(f"libfiphip error {rc}: {lib().fip_last_error().decode()}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Sim:
+    """One simulation on one GPU, created from a workload Case."""
+
+    def __init__(self, case: Case, mode: str = "dense", variant: str = "exact",
+                 tile_rows: int = 64, device: int = 0, staggered: bool = True,
+                 generate_bcs: bool = True):
+        L = lib()
+        self.case = case
+        self.W, self.H = case.W, case.H
+        keep = []
+
+        def arr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            keep.append(a)
+            return a
+
+        bed, h, qx, qy, kv = (arr(case.bed), arr(case.h), arr(case.qx), arr(case.qy),
+                              arr(case.kst_var))
+        d = FipDesc(case.W, case.H, case.dx, case.dt,
+                    case.no_data if case.no_data is not None else -9999.0, case.kst,
+                    _ptr(bed), _ptr(h), _ptr(qx), _ptr(qy), _ptr(kv),
+                    1 if case.variable_dt else 0, 1 if staggered else 0,
+                    1 if mode == "masked" else 0, 1 if variant == "fast" else 0, tile_rows, device)
+        s = C.c_void_p()
+        _check(L.fip_create(C.byref(d), C.byref(s)))
+        self.s = s
+        if generate_bcs:
+            self.bc_generate(case.bcs)
+
+    def close(self):
+        if getattr(self, "s", None):
+            lib().fip_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bc_generate(self, bcs):
+        arr = (FipBC * max(1, len(bcs)))()
+        self._series = []
+        for i, bc in enumerate(bcs):
+            frm, to = self.case.bc_range(bc)
+            b = arr[i]
+            b.type, b.side, b.from_, b.to = BC_TYPES[bc.type], BC_SIDES[bc.side], frm, to
+            b.z = bc.z if bc.z is not None else 0.0
+            b.q = bc.q if bc.q is not None else 0.0
+            for nm, ser in (("z", bc.z_series), ("q", bc.q_series)):
+                if ser is not None:
+                    t = np.asarray(ser[0], dtype=np.float32)
+                    v = np.asarray(ser[1], dtype=np.float32)
+                    self._series += [t, v]
+                    setattr(b, nm + "t", _ptr(t))
+                    setattr(b, nm + "v", _ptr(v))
+                    setattr(b, nm + "n", len(t))
+        _check(lib().fip_bc_generate(self.s, arr, len(bcs)))
+        self.nbc = len(bcs)
+
+    def bc_launch(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib().fip_bc_launch(self.s, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def bc_apply(self):
+        _check(lib().fip_bc_apply(self.s))
+
+    def bc_params(self):
+        p = np.zeros(2 * max(1, self.nbc), dtype=np.float32)
+        _check(lib().fip_bc_params(self.s, p.ctypes.data_as(C.POINTER(C.c_float)), self.nbc))
+        return p[:2 * self.nbc].reshape(-1, 2)
+
+    def control(self):
+        _check(lib().fip_control_timestep(self.s))
+
+    def compute_step(self):
+        _check(lib().fip_compute_step(self.s))
+
+    def floodplain(self):
+        _check(lib().fip_floodplain(self.s))
+
+    def step_reference_order(self):
+        """One simulate() iteration through the fine-grained seam (src/main.cpp:100-135)."""
+        t, dt, it = self.time()
+        if self.case.variable_dt:
+            self.control()
+        if it == 0:
+            self.bc_apply()
+        self.compute_step()
+        self.bc_apply()
+        if (it + 1) % 100 == 0:
+            self.floodplain()
+
+    def set_run(self, target, sampling_interval=0, points=(), save_times=()):
+        n = len(points)
+        xs = (C.c_int * max(1, n))(*[p[0] for p in points])
+        ys = (C.c_int * max(1, n))(*[p[1] for p in points])
+        ts = (C.c_int * max(1, n))(*[p[2] for p in points])
+        sv = (C.c_double * max(1, len(save_times)))(*save_times)
+        _check(lib().fip_set_run(self.s, float(target), int(sampling_interval), xs, ys, ts, n, sv,
+                                 len(save_times)))
+        self.npoints = n
+
+    def simulate(self, nsteps):
+        ev, done = C.c_int(), C.c_longlong()
+        _check(lib().fip_simulate(self.s, int(nsteps), C.byref(ev), C.byref(done)))
+        return ev.value, done.value
+
+    def read_samples(self, max_rows=256):
+        t = np.zeros(max_rows, dtype=np.float64)
+        v = np.zeros(max_rows * max(1, self.npoints), dtype=np.float32)
+        n = C.c_int()
+        _check(lib().fip_read_samples(self.s, t.ctypes.data_as(C.POINTER(C.c_double)),
+                                      v.ctypes.data_as(C.POINTER(C.c_float)), max_rows, C.byref(n)))
+        return t[:n.value], v[:n.value * self.npoints].reshape(n.value, self.npoints)
+
+    def time(self):
+        t, dt, it = C.c_double(), C.c_float(), C.c_int()
+        _check(lib().fip_get_time(self.s, C.byref(t), C.byref(dt), C.byref(it)))
+        return t.value, dt.value, it.value
+
+    def set_dt(self, dt):
+        _check(lib().fip_set_dt(self.s, dt))
+
+    def field(self, name, side=0):
+        out = np.empty((self.H, self.W), dtype=np.float32)
+        _check(lib().fip_download(self.s, FIELDS[name], side, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_field(self, name, side, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        _check(lib().fip_upload(self.s, FIELDS[name], side, a.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def state(self, side=0):
+        return {k: self.field(k, side) for k in ("h", "qx", "qy")}
+
+    def tiles(self):
+        a = [C.c_int() for _ in range(4)]
+        _check(lib().fip_tiles(self.s, *[C.byref(x) for x in a]))
+        return tuple(x.value for x in a)
+
+    def masks(self):
+        tx, ty, _, _ = self.tiles()
+        out = np.empty(tx * ty, dtype=np.uint64)
+        _check(lib().fip_get_masks(self.s, out.ctypes.data_as(C.POINTER(C.c_uint64)), len(out)))
+        return out
+
+    def cfl(self):
+        c = C.c_float()
+        _check(lib().fip_get_cfl(self.s, C.byref(c)))
+        return c.value
+
+    def set_timing(self, on: bool):
+        _check(lib().fip_set_timing(self.s, 1 if on else 0))
+
+    def stats(self):
+        st = FipStats()
+        _check(lib().fip_get_stats(self.s, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in FipStats._fields_}
+
+    def sync(self):
+        _check(lib().fip_sync(self.s))
