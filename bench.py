@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark of the fip time step on one MI355X.
+
+Workload: the plain large domain at the real shape (5667 x 6500, dx = 3 m, kSt_var,
+no_data polygon, left type-q hydrograph, right open BC), started from a state with ~30 %
+of the cells wet. A step is one simulate() iteration: the stencil kernel plus the
+post-step boundary/control kernel (and the flood-plain kernel every 100 steps), driven by
+the device-resident loop in hipGraph chunks. Inputs are resident in HBM before timing.
+
+Prints one JSON line: value = W*H*steps / wall seconds (Gcell-updates/s) in dense mode,
+the roofline of the stencil kernel (algorithmic 32 B per cell with kSt_var, timed with HIP
+events on the simulation's stream), the masked-mode effective rate, the exact-variant
+rate and the CPU oracle on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {False: 28, True: 32}  # uniform kSt / kSt_var
+
+
+def wet_state(case, frac=0.30):
+    """Flat water level that wets `frac` of the valid cells (the late state of the run)."""
+    valid = case.bed != np.float32(case.no_data if case.no_data is not None else -9999.0)
+    level = np.float32(np.quantile(case.bed[valid], frac))
+    h = np.where(valid, np.maximum(np.float32(0), level - case.bed), np.float32(0)).astype(np.float32)
+    case.h = h
+    return case
+
+
+def run_rate(case, mode, variant, rows, steps, warmup, timed_steps):
+    from fip_amd.sim import Sim
+    s = Sim(case, mode=mode, variant=variant, tile_rows=rows)
+    s.set_run(1e30)
+    s.simulate(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    ev, done = s.simulate(steps)
+    s.sync()
+    t1 = time.perf_counter()
+    assert done == steps, (ev, done)
+    st = s.stats()
+    tiles = st["tiles_computed"], st["tiles_total"]
+    # stencil kernel time: HIP events around each launch on the simulation's stream
+    s.set_timing(True)
+    s0 = s.stats()
+    s.simulate(timed_steps)
+    s1 = s.stats()
+    s.set_timing(False)
+    ms = (s1["stencil_ms"] - s0["stencil_ms"]) / max(1, s1["timed_launches"] - s0["timed_launches"])
+    s.close()
+    return (t1 - t0), ms, tiles
+
+
+def cpu_baseline(case, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import DENSE, Oracle
+    o = Oracle(case, DENSE, threads=threads)
+    o.step()  # first-touch and BC(0) outside the timed sample
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return case.W * case.H * steps / dt / 1e9, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--rows", type=int, default=64)
+    ap.add_argument("--timed-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the single-domain step does not shard "
+              "across GPUs here (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from fip_amd.workloads import config1
+    case = wet_state(config1())
+    W, H = case.W, case.H
+    kvar = case.kst_var is not None
+    bpc = BYTES_PER_CELL[kvar]
+
+    wall, ms, _ = run_rate(case, "dense", args.variant, args.rows, args.steps, args.warmup,
+                           args.timed_steps)
+    value = W * H * args.steps / wall / 1e9
+    achieved = bpc * W * H / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "Gcell-updates/s + % HBM roofline; learned-vs-plain speedup at depth RMSE<=2cm",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded terrain, Voronoi kSt, no_data polygon, q hydrograph)",
+        "config": {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q BC, "
+                               "right open, 30% wet, dense mode",
+                   "W": W, "H": H, "mode": "dense", "variant": args.variant,
+                   "tile": f"60x{args.rows}", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "stencil_dense", "kernel_ms": round(ms, 4),
+                     "bytes_per_cell": bpc},
+    }
+    if not args.no_extra:
+        other = "exact" if args.variant == "fast" else "fast"
+        w2, ms2, _ = run_rate(case, "dense", other, args.rows, max(100, args.steps // 4),
+                              args.warmup, args.timed_steps)
+        n2 = max(100, args.steps // 4)
+        out[f"dense_{other}"] = {"gcells": round(W * H * n2 / w2 / 1e9, 3),
+                                 "kernel_ms": round(ms2, 4),
+                                 "roofline_frac": round(bpc * W * H / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        w3, ms3, tiles = run_rate(case, "masked", args.variant, args.rows, max(100, args.steps // 4),
+                                  args.warmup, args.timed_steps)
+        n3 = max(100, args.steps // 4)
+        out["masked"] = {"gcells_effective": round(W * H * n3 / w3 / 1e9, 3),
+                         "kernel_ms": round(ms3, 4), "tiles_active": tiles[0],
+                         "tiles_total": tiles[1]}
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rate, secs = cpu_baseline(case, args.cpu_steps, threads)
+        out["cpu_baseline"] = {"value": round(rate, 5), "unit": "Gcell-updates/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{args.cpu_steps} dense steps of the full 5667x6500 "
+                                         f"grid, binary32 oracle, {secs:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
