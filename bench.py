@@ -73,6 +73,23 @@ def cpu_baseline(case, steps, threads):
     return case.W * case.H * steps / dt / 1e9, dt
 
 
+def calibrate_copy(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size: the reference kernel for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,7 +101,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run 1 GiB device copies (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the single-domain step does not shard "
               "across GPUs here (run with --gpus 1)", file=sys.stderr)

@@ -28,7 +28,7 @@ EMBED = ["fip_ctl.h", "fip_detmath.h", "fip_sched.h", "fip_step_common.h"]
 
 UNITS = [
     ("stencil_exact.hip", ["-ffp-contract=off"]),
-    ("stencil_fast.hip", ["-ffp-contract=fast"]),
+    ("stencil_fast.hip", ["-ffp-contract=fast", "-ffinite-math-only"]),
     ("kernels_misc.hip", ["-ffp-contract=off"]),
     ("fip_lib.cpp", ["-ffp-contract=off", "-x", "hip"]),
 ]

@@ -1,6 +1,7 @@
 #include "yaml_lite.h"
 
 #include <cctype>
+#include <cstring>
 #include <fstream>
 #include <limits>
 #include <sstream>

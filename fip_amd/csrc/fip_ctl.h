@@ -64,6 +64,8 @@ typedef struct fip_grid {
   const float* series;        // BC time-series tables
   fip_ctl* ctl;
   int W, H, P;            // P = row stride in floats; element (i, j) is at [j * P + i]
+  int margin;             // elements from the start of an allocation to element (0, 0)
+  int field_bytes;        // bytes per field allocation (buffer-resource range)
   int tiles_x, tiles_y, tile_rows;
   float dx, no_data, kst_uniform;
 } fip_grid;

@@ -452,10 +452,14 @@ int fip_create(const fip_desc* d, fip_sim** out) {
   const int rows = H + 2 * kMarginRows;
   s->field_elems = (size_t)P * rows;
   s->interior_off = (long)kMarginRows * P + kMarginCols;
+  if (s->field_elems * sizeof(float) > (size_t)0x7fffffff)
+    return cleanup(fail(FIP_E_INVALID, "grid too large: a field must stay below 2 GiB"));
   fip_grid& g = s->g;
   g.W = W;
   g.H = H;
   g.P = P;
+  g.margin = (int)s->interior_off;
+  g.field_bytes = (int)(s->field_elems * sizeof(float));
   g.tile_rows = R;
   g.tiles_x = (W + kTileCols - 1) / kTileCols;
   g.tiles_y = (H + R - 1) / R;
@@ -793,6 +797,14 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
     if (r & FIP_HALT_FINISHED) {
       *event = FIP_EV_FINISHED;
       break;
+    }
+    if (s->npoints > 0) {
+      int rows = 0;
+      if ((rc = ctl_read(s, CTL_OFF(nsample_rows), &rows))) return rc;
+      if (rows > FIP_SAMPLE_LOG_ROWS - kStepsPerGraph - 1) {
+        *event = FIP_EV_SAMPLES;
+        break;
+      }
     }
   }
   if (steps_done) *steps_done = done;

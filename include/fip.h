@@ -63,6 +63,7 @@ extern "C" {
 #define FIP_EV_NONE 0
 #define FIP_EV_FINISHED 1  /* duration >= target duration */
 #define FIP_EV_DUMP 2      /* a save_state time was reached; dump FIP_SIDE_OTHER + hmax */
+#define FIP_EV_SAMPLES 3   /* the device sample log is nearly full: drain it, then continue */
 
 typedef struct fip_sim fip_sim;
 
