@@ -103,6 +103,9 @@ def build_cli(verbose: bool = False) -> str | None:
 def build_oracle(verbose: bool = False) -> str:
     d = os.path.join(ROOT, "oracle")
     subprocess.run(["make", "-s", "-C", d, "-B"], check=True)
+    # the reference's own host sources, compiled where they lie, when that tree is present
+    if os.path.isfile("/root/reference/src/timeseries.cpp"):
+        subprocess.run(["make", "-s", "-C", d, "-B", "ref"], check=True)
     if verbose:
         print("built", os.path.join(d, "liboracle.so"))
     return os.path.join(d, "liboracle.so")

@@ -34,6 +34,7 @@ typedef struct fip_ctl {
   int nsample_rows;      // rows written to the sample log since the host last drained it
   int nsample_points;
   int sample_overflow;
+  int pre_sampled;       // the pre-first-step kernel logged the sample of the first call
   // active-tile schedule (masked mode), double-buffered by iteration parity
   unsigned list_count[2];
   unsigned work_next[2];

@@ -74,7 +74,9 @@ struct fip_sim {
   int sched_blocks = 0;
   // boundary kernel
   hipModule_t module = nullptr;
-  hipFunction_t bc_fn = nullptr;
+  hipFunction_t bc_post_fn = nullptr;
+  hipFunction_t bc_apply_fn = nullptr;
+  std::vector<int> bc_stage_items;  // border positions per apply stage
   int nbc = 0;
   bool bc_ready = false;
   float* d_series = nullptr;
@@ -123,10 +125,20 @@ enum { BC_PRE0 = 0, BC_POST = 1, BC_APPLY_DEVICE = 2, BC_APPLY_HOST = 3 };
 
 int launch_bc(fip_sim* s, int mode, int par) {
   if (!s->bc_ready) return fail(FIP_E_STATE, "boundary kernel not generated (call fip_bc_generate)");
-  int blocks = 1;
-  if (s->masked && (mode == BC_PRE0 || mode == BC_POST)) blocks += s->sched_blocks;
-  void* args[] = {&s->g, &mode, &par};
-  HIPTRY(hipModuleLaunchKernel(s->bc_fn, blocks, 1, 1, 1024, 1, 1, 0, s->stream, args, nullptr));
+  if (mode != BC_APPLY_HOST) {
+    void* args[] = {&s->g, &mode, &par};
+    HIPTRY(hipModuleLaunchKernel(s->bc_post_fn, 1, 1, 1, 256, 1, 1, 0, s->stream, args, nullptr));
+  }
+  const int sched = (s->masked && (mode == BC_PRE0 || mode == BC_POST)) ? s->sched_blocks : 0;
+  const int nst = std::max<int>(1, (int)s->bc_stage_items.size());
+  for (int st = 0; st < nst; st++) {
+    const int items = st < (int)s->bc_stage_items.size() ? s->bc_stage_items[st] : 0;
+    int napply = (items + 255) / 256;
+    const int blocks = napply + (st == 0 ? sched : 0);
+    if (blocks == 0) continue;
+    void* args[] = {&s->g, &mode, &par, (void*)&st, &napply};
+    HIPTRY(hipModuleLaunchKernel(s->bc_apply_fn, blocks, 1, 1, 256, 1, 1, 0, s->stream, args, nullptr));
+  }
   return FIP_OK;
 }
 
@@ -146,186 +158,327 @@ int ctl_read(fip_sim* s, size_t off, T* v, size_t n = 1) {
 
 #define CTL_OFF(f) offsetof(fip_ctl, f)
 
-// Source of the boundary kernel for this BC list: the counterpart of the string templates
-// of src/bc_kernel_gen.cpp:11-152. One work-group applies the BCs in list order with a
-// barrier between them, so corner cells shared by two BCs get a defined result. Modes:
-//   PRE0  - before the first step: controlTimestep, sample, BC(t=0) (main.cpp:102-114)
-//   POST  - after a stencil step of parity par: duration += dt, iteration++, swap, BC(t),
-//           save_state / end-of-run checks, then controlTimestep and sample of the next
-//           simulate() call (main.cpp:116-135, 102-111)
-//   APPLY_DEVICE - BC(t) with parameters evaluated on the device
-//   APPLY_HOST   - BC with parameters uploaded by the host (BoundaryConditionKernel::launch)
-// Blocks other than 0 (masked mode, PRE0/POST) build the next active-tile list.
-std::string bc_source(const std::vector<BCDesc>& bcs, bool staggered, int W, int H) {
+// Boundary kernels for one BC list: the counterpart of the string templates of
+// src/bc_kernel_gen.cpp:11-152, split in two launches.
+//
+// fip_bc_post - one work-group: the serial per-step control. Modes:
+//   PRE0  - before the first step: controlTimestep, sample, BC parameters at t
+//           (src/main.cpp:102-114)
+//   POST  - after a stencil step of parity par: the sample of the simulate() call that
+//           step belongs to (taken now; the stencil does not write its input side),
+//           duration += dt, iteration++, BC parameters at the new t, save_state / end of
+//           run requests, controlTimestep of the next call (src/main.cpp:100-135)
+//   APPLY_DEVICE - BC parameters at t only
+// fip_bc_apply - many work-groups, one border position per thread: applies the BCs with
+//   the parameters in the control block (APPLY_HOST: uploaded by the host, as
+//   BoundaryConditionKernel::launch). The reference applies the list in order within one
+//   thread per position; BC j must run after every earlier BC whose border strip (the two
+//   cells deep at each position it reads or writes) intersects its own, so the list is cut
+//   into stages of pairwise-disjoint BCs, one launch each. Extra work-groups of the first
+//   stage build the next active-tile list (masked mode, PRE0/POST).
+// stage_items receives the number of border positions of each stage.
+std::string bc_source(const std::vector<BCDesc>& bcs, bool staggered, int W, int H,
+                      std::vector<int>* stage_items) {
   std::string s;
   s += "#include \"fip_ctl.h\"\n#include \"fip_detmath.h\"\n#include \"fip_sched.h\"\n"
        "#include \"fip_step_common.h\"\n\n";
+  // the reference launches one thread per position of the longer side (rounded up to 64),
+  // so positions beyond from + threads - 1 are never set (src/bc_kernel_gen.cpp:143-149)
   const int ntid = (std::max(W, H) / 64 + 1) * 64;
   char buf[1024];
-  for (size_t i = 0; i < bcs.size(); i++) {
-    const BCDesc& b = bcs[i];
-    if (b.type != FIP_BC_Q) continue;
-    // BoundaryCondition::sumWaterDepths (src/boundary_condition.cpp:6-18), cached per z.
-    const char* idx = b.side == FIP_SIDE_LEFT    ? "(long)n * g.P"
-                      : b.side == FIP_SIDE_RIGHT ? "(long)n * g.P + (g.W - 1)"
-                      : b.side == FIP_SIDE_TOP   ? "(long)(g.H - 1) * g.P + n"
-                                                 : "(long)n";
-    std::snprintf(buf, sizeof buf,
-                  "__device__ float fip_swd_%zu(const fip_grid& g, float z) {\n"
-                  "  fip_ctl* c = g.ctl;\n"
-                  "  if (c->swd_valid[%zu] && fip_f2u(c->swd_z[%zu]) == fip_f2u(z)) return c->swd_val[%zu];\n"
-                  "  float ah_sum = 0.f;\n"
-                  "  for (int n = %d; n < %d; n++) {\n"
-                  "    float ah = z - g.bed[%s];\n"
-                  "    if (ah < 0.01f) ah = 0.f;\n"
-                  "    ah_sum += __builtin_sqrtf(9.81f * ah) * ah;\n"
-                  "  }\n"
-                  "  ah_sum *= g.dx;\n"
-                  "  c->swd_z[%zu] = z; c->swd_val[%zu] = ah_sum; c->swd_valid[%zu] = 1;\n"
-                  "  return ah_sum;\n}\n\n",
-                  i, i, i, i, b.from, b.to, idx, i, i, i);
-    s += buf;
-  }
-  s += "extern \"C\" __global__ __launch_bounds__(1024) void fip_bc_kernel(fip_grid g, int mode, int par) {\n"
+  const int n = (int)bcs.size();
+  const int nb = std::max(1, n);
+  int nser = 0;
+  for (const BCDesc& b : bcs) nser = std::max(nser, std::max(b.zoff + 2 * b.zn, b.qoff + 2 * b.qn));
+  const bool ser_lds = nser > 0 && nser <= 8192;
+  bool anyq = false;
+  for (const BCDesc& b : bcs) anyq |= b.type == FIP_BC_Q;
+
+  // ---- fip_bc_post ----
+  s += "extern \"C\" __global__ __launch_bounds__(256) void fip_bc_post(fip_grid g, int mode, int par) {\n"
        "  fip_ctl* c = g.ctl;\n"
-       "  if (blockIdx.x > 0) {\n"
-       "    if (mode == 0) { fip_schedule(g, 0, blockIdx.x - 1, gridDim.x - 1); return; }\n"
-       "    if (mode == 1 && !c->halt_req[par ^ 1]) fip_schedule(g, par ^ 1, blockIdx.x - 1, gridDim.x - 1);\n"
-       "    return;\n"
-       "  }\n";
+       "  const int tid = threadIdx.x;\n";
   std::snprintf(buf, sizeof buf,
-                "  __shared__ float s_q[%d], s_z[%d];\n  __shared__ int s_skip, s_cur;\n",
-                (int)std::max<size_t>(1, bcs.size()), (int)std::max<size_t>(1, bcs.size()));
+                "  __shared__ int s_skip, s_row, s_np, s_cur;\n"
+                "  __shared__ int s_need[%d];\n  __shared__ float s_zv[%d], s_swd[%d];\n",
+                nb, nb, nb);
   s += buf;
-  s += "  if (threadIdx.x == 0) {\n"
-       "    int skip = 0, cur = par;\n"
+  if (anyq) s += "  __shared__ float s_terms[256];\n";
+  if (ser_lds) {
+    std::snprintf(buf, sizeof buf,
+                  "  __shared__ float s_series[%d];\n"
+                  "  for (int i = tid; i < %d; i += blockDim.x) s_series[i] = g.series[i];\n"
+                  "  const float* ser = s_series;\n",
+                  nser, nser);
+    s += buf;
+  } else {
+    s += "  const float* ser = g.series;\n";
+  }
+  s += "  (void)ser;\n"
+       "  double dur = 0.0, target = 0.0, timer = 0.0, next_save = 0.0;\n"
+       "  float dt = 0.f, cflp = 0.f;\n"
+       "  unsigned acc = 0u;\n"
+       "  int it = 0, vdt = 0, interval = 0, spos = 0, nsave = 0, hreq = 0, pre = 0, nrows = 0, np = 0;\n";
+  for (int i = 0; i < n; i++)
+    if (bcs[i].type == FIP_BC_Q) {
+      std::snprintf(buf, sizeof buf, "  int swd_ok%d = 0; float swd_z%d = 0.f, swd_v%d = 0.f;\n", i, i, i);
+      s += buf;
+    }
+  // one batch of loads: no store of this thread precedes them
+  s += "  if (tid == 0) {\n"
+       "    hreq = c->halt_req[par ^ 1]; dur = c->duration; dt = c->dt; it = c->iteration;\n"
+       "    acc = c->cfl_acc[par]; cflp = c->cfl_prev; vdt = c->variable_dt; target = c->target;\n"
+       "    timer = c->sample_timer; interval = c->sampling_interval; spos = c->save_pos;\n"
+       "    nsave = c->nsave; next_save = c->save_times[spos < FIP_MAX_SAVE ? spos : 0];\n"
+       "    pre = c->pre_sampled; nrows = c->nsample_rows; np = c->nsample_points;\n";
+  for (int i = 0; i < n; i++)
+    if (bcs[i].type == FIP_BC_Q) {
+      std::snprintf(buf, sizeof buf,
+                    "    swd_ok%d = c->swd_valid[%d]; swd_z%d = c->swd_z[%d]; swd_v%d = c->swd_val[%d];\n",
+                    i, i, i, i, i, i);
+      s += buf;
+    }
+  s += "  }\n  __syncthreads();\n"
+       "  if (tid == 0) {\n"
+       "    int skip = 0, row = -1, cur = par;\n"
        "    if (mode == 1) {\n"
-       "      if (c->halt_req[par ^ 1]) { c->halt_req[par] = c->halt_req[par ^ 1]; skip = 1; }\n"
+       "      if (hreq) { c->halt_req[par] = hreq; skip = 1; }\n"
        "      else {\n"
-       "        c->duration = c->duration + (double)c->dt;\n"
-       "        c->iteration = c->iteration + 1;\n"
-       "        c->cfl_prev = fip_u2f(c->cfl_acc[par]);\n"
-       "        c->cfl_acc[par] = fip_f2u(1.0f);\n"
-       "        c->list_count[par] = 0u;\n"
-       "        c->work_next[par] = 0u;\n"
-       "        cur = par ^ 1;\n"
+       "        if (pre) c->pre_sampled = 0;\n"
+       "        else row = fip_sample_begin(c, dur, timer, interval, nrows);\n"
+       "        dur = dur + (double)dt;\n"
+       "        it = it + 1;\n"
+       "        cflp = fip_u2f(acc);\n"
+       "        c->duration = dur; c->iteration = it; c->cfl_prev = cflp;\n"
+       "        c->cfl_acc[par] = fip_f2u(1.0f); c->list_count[par] = 0u; c->work_next[par] = 0u;\n"
        "      }\n"
        "    } else if (mode == 0) {\n"
-       "      if (c->variable_dt) fip_control(c);\n"
-       "      fip_sample(g, par);\n"
+       "      if (vdt) { dt = fip_control_dt(dt, cflp); c->dt = dt; }\n"
+       "      row = fip_sample_begin(c, dur, timer, interval, nrows);\n"
+       "      c->pre_sampled = 1;\n"
        "    }\n"
        "    if (!skip) {\n"
-       "      const float tf = (float)c->duration;\n"
+       "      const float tf = (float)dur;\n"
        "      (void)tf;\n";
-  for (size_t i = 0; i < bcs.size(); i++) {
+  // getZ of each BC at the new time (src/boundary_condition.cpp:66-77); a series BC
+  // without a z series reads z as 0
+  for (int i = 0; i < n; i++) {
     const BCDesc& b = bcs[i];
-    std::string zexpr, qexpr;
     const bool series = b.zn > 0 || b.qn > 0;
+    std::string zexpr;
     if (!series) {
       zexpr = hexf(b.z);
-      qexpr = b.type == FIP_BC_Q ? "(" + hexf(b.q) + " / fip_swd_" + std::to_string(i) + "(g, zv))"
-                                 : hexf(b.q);
+    } else if (b.zn > 0) {
+      std::snprintf(buf, sizeof buf, "fip_interp(ser + %d, ser + %d, %d, tf)", b.zoff, b.zoff + b.zn, b.zn);
+      zexpr = buf;
     } else {
-      // getZ/getQ of BoundaryConditionTimeseries (src/boundary_condition.cpp:66-77); a
-      // series BC without a z series reads z as 0.
-      if (b.zn > 0) {
-        std::snprintf(buf, sizeof buf, "fip_interp(g.series + %d, g.series + %d, %d, tf)", b.zoff,
-                      b.zoff + b.zn, b.zn);
-        zexpr = buf;
-      } else {
-        zexpr = "0.f";
-      }
-      if (b.type == FIP_BC_Q) {
-        std::string qs = "0.f";
-        if (b.qn > 0) {
-          std::snprintf(buf, sizeof buf, "fip_interp(g.series + %d, g.series + %d, %d, tf)",
-                        b.qoff, b.qoff + b.qn, b.qn);
-          qs = buf;
-        }
-        qexpr = "(" + qs + " / fip_swd_" + std::to_string(i) + "(g, zv))";
-      } else {
-        qexpr = "0.f";
-      }
+      zexpr = "0.f";
     }
+    std::snprintf(buf, sizeof buf, "      s_zv[%d] = %s;\n", i, zexpr.c_str());
+    s += buf;
+    if (b.type == FIP_BC_Q) {
+      std::snprintf(buf, sizeof buf,
+                    "      s_need[%d] = !(swd_ok%d && fip_f2u(swd_z%d) == fip_f2u(s_zv[%d]));\n", i, i,
+                    i, i);
+      s += buf;
+    }
+  }
+  s += "    }\n"
+       "    s_skip = skip; s_row = row; s_np = np; s_cur = cur;\n"
+       "  }\n  __syncthreads();\n"
+       "  if (s_skip) return;\n"
+       "  if (s_row >= 0 && tid < s_np) fip_sample_point(g, s_cur, s_row, tid);\n"
+       "  if (mode == 3) return;\n";
+  // BoundaryCondition::sumWaterDepths (src/boundary_condition.cpp:6-18) when z changed:
+  // the terms are formed in parallel, thread 0 adds them in the reference's order
+  for (int i = 0; i < n; i++) {
+    const BCDesc& b = bcs[i];
+    if (b.type != FIP_BC_Q) continue;
+    const char* idx = b.side == FIP_SIDE_LEFT    ? "(long)k * g.P"
+                      : b.side == FIP_SIDE_RIGHT ? "(long)k * g.P + (g.W - 1)"
+                      : b.side == FIP_SIDE_TOP   ? "(long)(g.H - 1) * g.P + k"
+                                                 : "(long)k";
     std::snprintf(buf, sizeof buf,
-                  "      if (mode == 3) { s_q[%zu] = c->bc_q[%zu]; s_z[%zu] = c->bc_z[%zu]; }\n"
-                  "      else { const float zv = %s; const float qv = %s;\n"
-                  "             s_q[%zu] = qv; s_z[%zu] = zv; c->bc_q[%zu] = qv; c->bc_z[%zu] = zv; }\n",
-                  i, i, i, i, zexpr.c_str(), qexpr.c_str(), i, i, i, i);
+                  "  if (s_need[%d]) {\n"
+                  "    const float zv = s_zv[%d];\n"
+                  "    float ah_sum = 0.f;\n"
+                  "    for (int base = %d; base < %d; base += 256) {\n"
+                  "      const int k = base + tid;\n"
+                  "      if (k < %d) {\n"
+                  "        float ah = zv - g.bed[%s];\n"
+                  "        if (ah < 0.01f) ah = 0.f;\n"
+                  "        s_terms[tid] = __builtin_sqrtf(9.81f * ah) * ah;\n"
+                  "      }\n"
+                  "      __syncthreads();\n"
+                  "      if (tid == 0) {\n"
+                  "        const int m = %d - base < 256 ? %d - base : 256;\n"
+                  "        for (int e = 0; e < m; e++) ah_sum += s_terms[e];\n"
+                  "      }\n"
+                  "      __syncthreads();\n"
+                  "    }\n"
+                  "    if (tid == 0) s_swd[%d] = ah_sum * g.dx;\n"
+                  "  }\n",
+                  i, i, b.from, b.to, b.to, idx, b.to, b.to, i);
     s += buf;
   }
-  s += "    }\n    s_skip = skip;\n    s_cur = cur;\n  }\n  __syncthreads();\n"
-       "  if (s_skip) return;\n"
-       "  const int cur = s_cur;\n"
-       "  float* h = g.h[cur];\n  float* qx = g.qx[cur];\n  float* qy = g.qy[cur];\n"
-       "  const float* sohle = g.bed;\n  (void)h; (void)qx; (void)qy; (void)sohle;\n";
-  for (size_t i = 0; i < bcs.size(); i++) {
+  s += "  if (tid != 0) return;\n";
+  for (int i = 0; i < n; i++) {
     const BCDesc& b = bcs[i];
-    const char *idx, *stag, *nb, *q, *ql;
-    if (b.side == FIP_SIDE_RIGHT) {
-      idx = "(long)tid * g.P + (g.W - 1)"; stag = "idx - 1"; nb = "idx - 1"; q = "qx"; ql = "qy";
-    } else if (b.side == FIP_SIDE_LEFT) {
-      idx = "(long)tid * g.P"; stag = "idx"; nb = "idx + 1"; q = "qx"; ql = "qy";
-    } else if (b.side == FIP_SIDE_TOP) {
-      idx = "(long)tid + (long)(g.H - 1) * g.P"; stag = "idx - g.P"; nb = "idx - g.P"; q = "qy"; ql = "qx";
+    const bool series = b.zn > 0 || b.qn > 0;
+    std::string qexpr;
+    if (b.type == FIP_BC_Q) {
+      std::snprintf(buf, sizeof buf,
+                    "  {\n    float swd = swd_v%d;\n"
+                    "    if (s_need[%d]) { swd = s_swd[%d]; c->swd_z[%d] = s_zv[%d]; c->swd_val[%d] = swd; c->swd_valid[%d] = 1; }\n",
+                    i, i, i, i, i, i, i);
+      s += buf;
+      std::string qs;
+      if (!series) {
+        qs = hexf(b.q);
+      } else if (b.qn > 0) {
+        std::snprintf(buf, sizeof buf, "fip_interp(ser + %d, ser + %d, %d, (float)dur)", b.qoff,
+                      b.qoff + b.qn, b.qn);
+        qs = buf;
+      } else {
+        qs = "0.f";
+      }
+      qexpr = "(" + qs + " / swd)";
     } else {
-      idx = "(long)tid"; stag = "idx"; nb = "idx + g.P"; q = "qy"; ql = "qx";
+      s += "  {\n";
+      qexpr = series ? "0.f" : hexf(b.q);
     }
+    std::snprintf(buf, sizeof buf, "    c->bc_q[%d] = %s; c->bc_z[%d] = s_zv[%d];\n  }\n", i,
+                  qexpr.c_str(), i, i);
+    s += buf;
+  }
+  s += "  if (mode == 1) {\n"
+       "    int req = 0;\n"
+       "    if (spos < nsave && dur >= next_save) { c->save_pos = spos + 1; req |= FIP_HALT_DUMP; }\n"
+       "    if (!(dur < target)) req |= FIP_HALT_FINISHED;\n"
+       "    c->halt_req[par] = req;\n"
+       "    if (!(req & FIP_HALT_FINISHED) && vdt) c->dt = fip_control_dt(dt, cflp);\n"
+       "  }\n"
+       "}\n\n";
+
+  // ---- fip_bc_apply ----
+  std::vector<int> stage(n, 0), lim(n);
+  auto strip = [&](int i, int* x0, int* x1, int* y0, int* y1) {
+    const BCDesc& b = bcs[i];
+    if (b.side == FIP_SIDE_LEFT) { *x0 = 0; *x1 = 1; *y0 = b.from; *y1 = lim[i]; }
+    else if (b.side == FIP_SIDE_RIGHT) { *x0 = W - 2; *x1 = W - 1; *y0 = b.from; *y1 = lim[i]; }
+    else if (b.side == FIP_SIDE_TOP) { *x0 = b.from; *x1 = lim[i]; *y0 = H - 2; *y1 = H - 1; }
+    else { *x0 = b.from; *x1 = lim[i]; *y0 = 0; *y1 = 1; }
+  };
+  for (int i = 0; i < n; i++) lim[i] = std::min(bcs[i].to, bcs[i].from + ntid - 1);
+  int nstages = 0;
+  for (int j = 0; j < n; j++) {
+    if (lim[j] < bcs[j].from) continue;
+    int a0, a1, b0, b1;
+    strip(j, &a0, &a1, &b0, &b1);
+    for (int i = 0; i < j; i++) {
+      if (lim[i] < bcs[i].from) continue;
+      int c0, c1, d0, d1;
+      strip(i, &c0, &c1, &d0, &d1);
+      if (a0 <= c1 && c0 <= a1 && b0 <= d1 && d0 <= b1) stage[j] = std::max(stage[j], stage[i] + 1);
+    }
+    nstages = std::max(nstages, stage[j] + 1);
+  }
+  s += "extern \"C\" __global__ __launch_bounds__(256) void fip_bc_apply(fip_grid g, int mode, int par,\n"
+       "                                                               int stage, int napply) {\n"
+       "  fip_ctl* c = g.ctl;\n"
+       "  if ((int)blockIdx.x >= napply) {\n"
+       "    const int blk = blockIdx.x - napply, nblk = gridDim.x - napply;\n"
+       "    if (mode == 0) fip_schedule(g, par, blk, nblk);\n"
+       "    else if (mode == 1 && !c->halt_req[par ^ 1]) fip_schedule(g, par ^ 1, blk, nblk);\n"
+       "    return;\n"
+       "  }\n"
+       "  if (mode == 1 && c->halt_req[par ^ 1]) return;\n"
+       "  const int cur = mode == 1 ? (par ^ 1) : par;\n"
+       "  float* h = g.h[cur];\n  float* qx = g.qx[cur];\n  float* qy = g.qy[cur];\n"
+       "  const float* sohle = g.bed;\n"
+       "  const int item = blockIdx.x * blockDim.x + threadIdx.x;\n"
+       "  int which = -1, tid = 0, side = 0;\n"
+       "  switch (stage) {\n";
+  if (stage_items) stage_items->assign(nstages, 0);
+  for (int st = 0; st < nstages; st++) {
+    int total = 0;
+    for (int i = 0; i < n; i++)
+      if (stage[i] == st && lim[i] >= bcs[i].from) total += lim[i] - bcs[i].from + 1;
+    std::snprintf(buf, sizeof buf, "  case %d:  // stage %d: %d border positions\n", st, st, total);
+    s += buf;
+    total = 0;
+    bool first = true;
+    for (int i = 0; i < n; i++) {
+      if (stage[i] != st || lim[i] < bcs[i].from) continue;
+      const int cnt = lim[i] - bcs[i].from + 1;
+      std::snprintf(buf, sizeof buf,
+                    "    %sif (item < %d) { which = %d; tid = %d + (item - %d); side = %d; }\n",
+                    first ? "" : "else ", total + cnt, i, bcs[i].from, total, bcs[i].side);
+      s += buf;
+      first = false;
+      total += cnt;
+    }
+    if (stage_items) (*stage_items)[st] = total;
+    s += "    break;\n";
+  }
+  s += "  default: break;\n  }\n"
+       "  if (which < 0) return;\n"
+       "  long idx, nb, stag;\n"
+       "  if (side == 1) { idx = (long)tid * g.P + (g.W - 1); stag = idx - 1; nb = idx - 1; }\n"
+       "  else if (side == 0) { idx = (long)tid * g.P; stag = idx; nb = idx + 1; }\n"
+       "  else if (side == 2) { idx = (long)tid + (long)(g.H - 1) * g.P; stag = idx - g.P; nb = idx - g.P; }\n"
+       "  else { idx = (long)tid; stag = idx; nb = idx + g.P; }\n"
+       "  (void)stag;\n"
+       "  const bool xface = side <= 1;  // q = qx, q_lateral = qy on the left/right sides\n"
+       "  const float hnb = h[nb], bnb = sohle[nb], bidx = sohle[idx];\n"
+       "  const float qlnb = xface ? qy[nb] : qx[nb];\n"
+       "  const float qnb = xface ? qx[nb] : qy[nb];\n"
+       "  (void)hnb; (void)bnb; (void)bidx; (void)qlnb; (void)qnb;\n"
+       "  const float qval = c->bc_q[which], zval = c->bc_z[which];\n"
+       "  (void)qval; (void)zval;\n"
+       "  switch (which) {\n";
+  for (int i = 0; i < n; i++) {
+    if (lim[i] < bcs[i].from) continue;
+    const BCDesc& b = bcs[i];
+    const std::string Q = (b.side <= FIP_SIDE_RIGHT) ? "qx" : "qy";
+    const std::string QL = (b.side <= FIP_SIDE_RIGHT) ? "qy" : "qx";
+    const std::string qi = staggered ? "stag" : "idx";
     // direction of the open-BC discharge: +1 on the right/top side, -1 on the left/bottom
     const char* sign = (b.side == FIP_SIDE_RIGHT || b.side == FIP_SIDE_TOP) ? "1.0f" : "(-1.0f)";
     std::string body;
-    const std::string Q = std::string(q), QL = std::string(ql);
-    const std::string qi = staggered ? "stag" : "idx";
     if (b.type == FIP_BC_Z) {
       if (staggered)
-        body = "h[idx] = fip_fmaxf(0.f, zval - sohle[idx]);\n"
-               "      h[nb] = fip_fmaxf(0.f, zval - sohle[nb]);\n"
+        body = "h[idx] = fip_fmaxf(0.f, zval - bidx);\n"
+               "      h[nb] = fip_fmaxf(0.f, zval - bnb);\n"
                "      " + Q + "[stag] = -0.f;\n";
       else
-        body = "float ah = fip_fmaxf(0.f, zval - sohle[idx]);\n"
+        body = "const float ah = fip_fmaxf(0.f, zval - bidx);\n"
                "      h[idx] = ah;\n"
-               "      " + Q + "[idx] = 0.99f * (" + Q + "[nb] / fip_fmaxf(h[nb], 0.01f)) * ah;\n";
+               "      " + Q + "[idx] = 0.99f * (qnb / fip_fmaxf(hnb, 0.01f)) * ah;\n";
     } else if (b.type == FIP_BC_Q) {
       if (staggered)
-        body = "float ah = zval - sohle[idx];\n      if (ah < 0.f) ah = 0.f;\n"
+        body = "float ah = zval - bidx;\n      if (ah < 0.f) ah = 0.f;\n"
                "      " + Q + "[stag] = __builtin_sqrtf(9.81f * ah) * ah * qval;\n"
-               "      if (ah > 0.f) { float z_neighbor = h[nb] + sohle[nb];\n"
-               "        h[idx] = fip_fmaxf(ah / 2, z_neighbor - sohle[idx]); }\n";
+               "      if (ah > 0.f) { const float z_neighbor = hnb + bnb;\n"
+               "        h[idx] = fip_fmaxf(ah / 2, z_neighbor - bidx); }\n";
       else
-        body = "float ah = zval - sohle[idx];\n      if (ah < 0.f) ah = 0.f;\n"
+        body = "float ah = zval - bidx;\n      if (ah < 0.f) ah = 0.f;\n"
                "      " + Q + "[idx] = __builtin_sqrtf(9.81f * ah) * ah * qval;\n"
-               "      if (ah > 0.f) h[idx] = fip_fmaxf(ah / 2, h[nb] + sohle[nb] - sohle[idx]);\n";
+               "      if (ah > 0.f) h[idx] = fip_fmaxf(ah / 2, hnb + bnb - bidx);\n";
     } else if (b.type == FIP_BC_CLOSE) {
-      body = Q + "[" + qi + "] = 0.f;\n      " + QL + "[idx] = " + QL + "[nb];\n      h[idx] = h[nb];\n";
+      body = Q + "[" + qi + "] = 0.f;\n      " + QL + "[idx] = qlnb;\n      h[idx] = hnb;\n";
     } else {
-      body = "float ah_neighbor = h[nb];\n      float z_neighbor = ah_neighbor + sohle[nb];\n"
+      body = "const float ah_neighbor = hnb;\n      const float z_neighbor = ah_neighbor + bnb;\n"
              "      if (ah_neighbor > 0.f) {\n"
              "        " + Q + "[" + qi + "] = " + sign + " * __builtin_sqrtf(9.81f * ah_neighbor) * ah_neighbor;\n"
-             "        h[idx] = fip_fminf(ah_neighbor, fip_fmaxf(0.f, z_neighbor - sohle[idx])); }\n";
+             "        h[idx] = fip_fminf(ah_neighbor, fip_fmaxf(0.f, z_neighbor - bidx)); }\n";
     }
-    const int lim = std::min(b.to, b.from + ntid - 1);
-    std::snprintf(buf, sizeof buf,
-                  "  { const float qval = s_q[%zu], zval = s_z[%zu]; (void)qval; (void)zval;\n"
-                  "    for (int tid = %d + (int)threadIdx.x; tid <= %d; tid += blockDim.x) {\n"
-                  "      const long idx = %s; const long stag = %s; const long nb = %s;\n"
-                  "      (void)stag;\n      ",
-                  i, i, b.from, lim, idx, stag, nb);
+    std::snprintf(buf, sizeof buf, "  case %d: {\n      ", i);
     s += buf;
     s += body;
-    s += "    }\n  }\n  __syncthreads();\n";
+    s += "      break; }\n";
   }
-  s += "  if (mode == 1 && threadIdx.x == 0) {\n"
-       "    int req = 0;\n"
-       "    if (c->save_pos < c->nsave && c->duration >= c->save_times[c->save_pos]) {\n"
-       "      c->save_pos = c->save_pos + 1; req |= FIP_HALT_DUMP; }\n"
-       "    if (!(c->duration < c->target)) req |= FIP_HALT_FINISHED;\n"
-       "    c->halt_req[par] = req;\n"
-       "    if (!(req & FIP_HALT_FINISHED)) {\n"
-       "      if (c->variable_dt) fip_control(c);\n"
-       "      fip_sample(g, cur);\n"
-       "    }\n"
-       "  }\n"
-       "}\n";
+  s += "  default: break;\n  }\n}\n";
   return s;
 }
 
@@ -361,7 +514,8 @@ int compile_bc(fip_sim* s, const std::string& src) {
     s->module = nullptr;
   }
   HIPTRY(hipModuleLoadData(&s->module, code.data()));
-  HIPTRY(hipModuleGetFunction(&s->bc_fn, s->module, "fip_bc_kernel"));
+  HIPTRY(hipModuleGetFunction(&s->bc_post_fn, s->module, "fip_bc_post"));
+  HIPTRY(hipModuleGetFunction(&s->bc_apply_fn, s->module, "fip_bc_apply"));
   return FIP_OK;
 }
 
@@ -589,7 +743,7 @@ int fip_bc_source(const fip_bc* bcs, int n, int W, int H, int staggered, char* b
   std::vector<float> table;
   int rc = collect_bcs(d, bcs, n, v, table);
   if (rc) return rc;
-  const std::string src = bc_source(v, staggered != 0, W, H);
+  const std::string src = bc_source(v, staggered != 0, W, H, nullptr);
   if (len) *len = src.size();
   if (buf && cap > 0) {
     const size_t k = std::min(cap - 1, src.size());
@@ -615,7 +769,9 @@ int fip_bc_generate(fip_sim* s, const fip_bc* bcs, int n) {
     HIPTRY(hipMemcpy(s->d_series, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   s->g.series = s->d_series;
-  rc = compile_bc(s, bc_source(v, s->d.staggered != 0, s->d.W, s->d.H));
+  std::vector<int> items;
+  rc = compile_bc(s, bc_source(v, s->d.staggered != 0, s->d.W, s->d.H, &items));
+  if (!rc) s->bc_stage_items = items;
   if (rc) return rc;
   // reset the sumWaterDepths cache
   int zeros[FIP_MAX_BC] = {0};
@@ -716,6 +872,7 @@ int fip_set_run(fip_sim* s, double target, int sampling_interval, const int* sx,
   hc->target = target;
   hc->sampling_interval = sampling_interval;
   hc->sample_timer = 0.0;  // Sampling::timer{duration} (src/simdata.cpp:41)
+  hc->pre_sampled = 0;
   hc->nsample_points = npoints;
   for (int p = 0; p < npoints; p++) {
     hc->sp_x[p] = sx[p];

@@ -5,18 +5,18 @@
 // controlTimestep (src/main.cpp:25-43), from the CFL minimum of the last step. The
 // reference scans a per-tile table whose entries start at 1; its minimum equals the
 // running minimum over wet stored cells (capped at 1) kept in cfl_prev.
-__device__ __forceinline__ void fip_control(fip_ctl* c) {
+__device__ __forceinline__ float fip_control_dt(float dt, float cfl_prev) {
   float min_cfl = 42.0f;
-  if (c->cfl_prev < min_cfl) min_cfl = c->cfl_prev;
+  if (cfl_prev < min_cfl) min_cfl = cfl_prev;
   const float target = min_cfl * .5f;
   const float dc = 0.001f;
-  float dt = c->dt;
   if (dt < target && target - dt > dc)
     dt *= 1.001f;
   else if (dt > target && dt - target > dc)
     dt *= 0.99f;
-  c->dt = dt;
+  return dt;
 }
+__device__ __forceinline__ void fip_control(fip_ctl* c) { c->dt = fip_control_dt(c->dt, c->cfl_prev); }
 
 // TimeSeries::interp (src/timeseries.cpp:14-26): std::map::lower_bound, clamped at both
 // ends, then f * upper + (1 - f) * lower.
@@ -37,28 +37,33 @@ __device__ __forceinline__ float fip_interp(const float* t, const float* v, int 
   return f * v[lo] + (1.f - f) * v[l];
 }
 
-// sample() (src/main.cpp:53-83) at the start of a simulate() call: logs (duration,
-// values) when duration >= Sampling::timer. Type z adds the bed to the depth.
-__device__ __forceinline__ void fip_sample(const fip_grid& g, int cur) {
-  fip_ctl* c = g.ctl;
-  if (!(c->duration >= c->sample_timer && c->sampling_interval > 0)) return;
-  c->sample_timer = c->sample_timer + (double)c->sampling_interval;
-  const int row = c->nsample_rows;
-  if (row >= FIP_SAMPLE_LOG_ROWS) {
+// sample() (src/main.cpp:53-83), run at the start of a simulate() call (main.cpp:108-112):
+// when duration >= Sampling::timer the timer advances by the interval and one row is
+// logged. fip_sample_begin runs on one thread, updates the timer and reserves the row
+// (returns -1 when nothing is logged); fip_sample_point then fills one point of it. Type
+// z adds the bed to the depth.
+__device__ __forceinline__ int fip_sample_begin(fip_ctl* c, double t, double timer, int interval,
+                                                int nrows) {
+  if (!(t >= timer && interval > 0)) return -1;
+  c->sample_timer = timer + (double)interval;
+  if (nrows >= FIP_SAMPLE_LOG_ROWS) {
     c->sample_overflow = 1;
-    return;
+    return -1;
   }
-  c->sample_t[row] = c->duration;
-  for (int p = 0; p < c->nsample_points; p++) {
-    const long i = (long)c->sp_y[p] * g.P + c->sp_x[p];
-    float v;
-    switch (c->sp_type[p]) {
-      case 0: v = g.h[cur][i]; v += g.bed[i]; break;
-      case 1: v = g.h[cur][i]; break;
-      case 2: v = g.qx[cur][i]; break;
-      default: v = g.qy[cur][i]; break;
-    }
-    c->sample_v[row * FIP_MAX_SAMPLE_POINTS + p] = v;
+  c->sample_t[nrows] = t;
+  c->nsample_rows = nrows + 1;
+  return nrows;
+}
+
+__device__ __forceinline__ void fip_sample_point(const fip_grid& g, int cur, int row, int p) {
+  fip_ctl* c = g.ctl;
+  const long i = (long)c->sp_y[p] * g.P + c->sp_x[p];
+  float v;
+  switch (c->sp_type[p]) {
+    case 0: v = g.h[cur][i]; v += g.bed[i]; break;
+    case 1: v = g.h[cur][i]; break;
+    case 2: v = g.qx[cur][i]; break;
+    default: v = g.qy[cur][i]; break;
   }
-  c->nsample_rows = row + 1;
+  c->sample_v[row * FIP_MAX_SAMPLE_POINTS + p] = v;
 }
