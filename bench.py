@@ -62,6 +62,18 @@ def run_rate(case, mode, variant, rows, steps, warmup, timed_steps):
     return (t1 - t0), ms, tiles
 
 
+def measured_traffic(variant, rows):
+    """HBM bytes per stencil launch from the committed rocprofv3 --pmc summary of this
+    workload (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_stencil_dense_{variant}_5667x6500_kstvar.md")
+    if rows != 64 or not os.path.isfile(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(case, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import DENSE, Oracle
@@ -98,7 +110,7 @@ def main():
     ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
     ap.add_argument("--rows", type=int, default=64)
     ap.add_argument("--timed-steps", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
     ap.add_argument("--calibrate", action="store_true",
@@ -122,6 +134,7 @@ def main():
                            args.timed_steps)
     value = W * H * args.steps / wall / 1e9
     achieved = bpc * W * H / (ms * 1e-3) / 1e9
+    traffic = measured_traffic(args.variant, args.rows)
     out = {
         "metric": "Gcell-updates/s + % HBM roofline; learned-vs-plain speedup at depth RMSE<=2cm",
         "value": round(value, 3),
@@ -140,9 +153,10 @@ def main():
                    "W": W, "H": H, "mode": "dense", "variant": args.variant,
                    "tile": f"60x{args.rows}", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "stencil_dense", "kernel_ms": round(ms, 4),
-                     "bytes_per_cell": bpc},
+                     "bytes_per_cell": bpc, "algorithmic_bytes": bpc * W * H,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, copy-calibrated"},
     }
     if not args.no_extra:
         other = "exact" if args.variant == "fast" else "fast"

@@ -8,6 +8,14 @@
 //   src/bc_kernel_gen.cpp:11-152 boundary-condition kernel bodies
 //   src/boundary_condition.cpp:6-77, src/timeseries.cpp:14-26 BC parameters
 // Real type: binary32 (bit-exact target of the HIP build) or binary64.
+//
+// Pinning: every function is checked by tests/test_oracle.py (three stencil restatements
+// that must agree, closed forms, invariants, a correctly-rounded pow sweep) and the host
+// series/expression code by tests/test_cli_host.py against the reference's own
+// timeseries.cpp compiled into oracle/_ref. Parity with the reference's CUDA binary
+// itself (fast-math __powf, approximate division and sqrt, flush-to-zero) is unpinned:
+// neither that binary nor a CUDA toolchain exists here, and the reference holds no
+// golden outputs.
 #pragma once
 #include <stdint.h>
 #include <stddef.h>
