@@ -24,13 +24,15 @@ NO_PK = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-Wno-unused-command-line-argument"] + NO_PK
 
-EMBED = ["fip_ctl.h", "fip_detmath.h", "fip_sched.h", "fip_step_common.h"]
+EMBED = ["fip_ctl.h", "fip_detmath.h", "fip_sched.h", "fip_step_common.h", "fip_bc_common.h"]
 
 UNITS = [
     ("stencil_exact.hip", ["-ffp-contract=off"]),
     ("stencil_fast.hip", ["-ffp-contract=fast", "-ffinite-math-only"]),
     ("kernels_misc.hip", ["-ffp-contract=off"]),
+    ("ensemble.hip", ["-ffp-contract=off"]),
     ("fip_lib.cpp", ["-ffp-contract=off", "-x", "hip"]),
+    ("fip_ens.cpp", ["-ffp-contract=off", "-x", "hip"]),
 ]
 
 

@@ -70,3 +70,36 @@ typedef struct fip_grid {
   int tiles_x, tiles_y, tile_rows;
   float dx, no_data, kst_uniform;
 } fip_grid;
+
+// ---- ensembles: many independent domains advanced by one launch per kernel and step ----
+#define FIP_ENS_MAX 1024
+
+// One scenario of an ensemble: its domain, its BC list [bc0, bc0 + nbc) and the index of
+// its first tile in the ensemble's global tile numbering.
+typedef struct fip_ens_scn {
+  fip_grid g;
+  int bc0, nbc;
+  int tile_base;
+  int staggered;
+} fip_ens_scn;
+
+// A boundary condition of an ensemble scenario (series offsets index the ensemble's
+// series table).
+typedef struct fip_ens_bc {
+  int type, side, from, to;
+  int zoff, zn, qoff, qn;
+  float z, q;
+} fip_ens_bc;
+
+// Up to 256 consecutive border positions of one BC, applied by one work-group.
+typedef struct fip_ens_chunk {
+  int scn, bc, pos0, n;
+} fip_ens_chunk;
+
+// Ensemble-wide schedule: the global active-tile list of each parity, entries
+// (scenario << 40) | (tile << 12) | (lo << 6) | hi.
+typedef struct fip_ens_ctl {
+  unsigned list_count[2];
+  unsigned work_next[2];
+  int finished[FIP_ENS_MAX];  // set by the post kernel when a scenario reached its target
+} fip_ens_ctl;

@@ -14,6 +14,7 @@
 #include "../../include/fip.h"
 #include "fip_ctl.h"
 #include "fip_embed.h"  // generated: the device headers as strings for hipRTC
+#include "fip_internal.h"
 
 hipError_t fip_launch_stencil_exact(const fip_grid& g, int par, int masked, int blocks,
                                     hipStream_t st);
@@ -181,10 +182,7 @@ std::string bc_source(const std::vector<BCDesc>& bcs, bool staggered, int W, int
                       std::vector<int>* stage_items) {
   std::string s;
   s += "#include \"fip_ctl.h\"\n#include \"fip_detmath.h\"\n#include \"fip_sched.h\"\n"
-       "#include \"fip_step_common.h\"\n\n";
-  // the reference launches one thread per position of the longer side (rounded up to 64),
-  // so positions beyond from + threads - 1 are never set (src/bc_kernel_gen.cpp:143-149)
-  const int ntid = (std::max(W, H) / 64 + 1) * 64;
+       "#include \"fip_step_common.h\"\n#include \"fip_bc_common.h\"\n\n";
   char buf[1024];
   const int n = (int)bcs.size();
   const int nb = std::max(1, n);
@@ -304,11 +302,7 @@ std::string bc_source(const std::vector<BCDesc>& bcs, bool staggered, int W, int
                   "    float ah_sum = 0.f;\n"
                   "    for (int base = %d; base < %d; base += 256) {\n"
                   "      const int k = base + tid;\n"
-                  "      if (k < %d) {\n"
-                  "        float ah = zv - g.bed[%s];\n"
-                  "        if (ah < 0.01f) ah = 0.f;\n"
-                  "        s_terms[tid] = __builtin_sqrtf(9.81f * ah) * ah;\n"
-                  "      }\n"
+                  "      if (k < %d) s_terms[tid] = fip_swd_term(zv, g.bed[%s]);\n"
                   "      __syncthreads();\n"
                   "      if (tid == 0) {\n"
                   "        const int m = %d - base < 256 ? %d - base : 256;\n"
@@ -361,28 +355,17 @@ std::string bc_source(const std::vector<BCDesc>& bcs, bool staggered, int W, int
        "}\n\n";
 
   // ---- fip_bc_apply ----
-  std::vector<int> stage(n, 0), lim(n);
-  auto strip = [&](int i, int* x0, int* x1, int* y0, int* y1) {
-    const BCDesc& b = bcs[i];
-    if (b.side == FIP_SIDE_LEFT) { *x0 = 0; *x1 = 1; *y0 = b.from; *y1 = lim[i]; }
-    else if (b.side == FIP_SIDE_RIGHT) { *x0 = W - 2; *x1 = W - 1; *y0 = b.from; *y1 = lim[i]; }
-    else if (b.side == FIP_SIDE_TOP) { *x0 = b.from; *x1 = lim[i]; *y0 = H - 2; *y1 = H - 1; }
-    else { *x0 = b.from; *x1 = lim[i]; *y0 = 0; *y1 = 1; }
-  };
-  for (int i = 0; i < n; i++) lim[i] = std::min(bcs[i].to, bcs[i].from + ntid - 1);
-  int nstages = 0;
-  for (int j = 0; j < n; j++) {
-    if (lim[j] < bcs[j].from) continue;
-    int a0, a1, b0, b1;
-    strip(j, &a0, &a1, &b0, &b1);
-    for (int i = 0; i < j; i++) {
-      if (lim[i] < bcs[i].from) continue;
-      int c0, c1, d0, d1;
-      strip(i, &c0, &c1, &d0, &d1);
-      if (a0 <= c1 && c0 <= a1 && b0 <= d1 && d0 <= b1) stage[j] = std::max(stage[j], stage[i] + 1);
+  std::vector<int> stage, lim;
+  {
+    std::vector<fip_bc> pub(n);
+    for (int i = 0; i < n; i++) {
+      pub[i] = fip_bc{};
+      pub[i].type = bcs[i].type; pub[i].side = bcs[i].side;
+      pub[i].from = bcs[i].from; pub[i].to = bcs[i].to;
     }
-    nstages = std::max(nstages, stage[j] + 1);
+    fip_internal::bc_stages(pub.data(), n, W, H, stage, lim);
   }
+  const int nstages = stage.empty() ? 0 : std::max(0, *std::max_element(stage.begin(), stage.end()) + 1);
   s += "extern \"C\" __global__ __launch_bounds__(256) void fip_bc_apply(fip_grid g, int mode, int par,\n"
        "                                                               int stage, int napply) {\n"
        "  fip_ctl* c = g.ctl;\n"
@@ -397,7 +380,7 @@ std::string bc_source(const std::vector<BCDesc>& bcs, bool staggered, int W, int
        "  float* h = g.h[cur];\n  float* qx = g.qx[cur];\n  float* qy = g.qy[cur];\n"
        "  const float* sohle = g.bed;\n"
        "  const int item = blockIdx.x * blockDim.x + threadIdx.x;\n"
-       "  int which = -1, tid = 0, side = 0;\n"
+       "  int which = -1, tid = 0, side = 0;\n  (void)side;\n"
        "  switch (stage) {\n";
   if (stage_items) stage_items->assign(nstages, 0);
   for (int st = 0; st < nstages; st++) {
@@ -423,60 +406,14 @@ std::string bc_source(const std::vector<BCDesc>& bcs, bool staggered, int W, int
   }
   s += "  default: break;\n  }\n"
        "  if (which < 0) return;\n"
-       "  long idx, nb, stag;\n"
-       "  if (side == 1) { idx = (long)tid * g.P + (g.W - 1); stag = idx - 1; nb = idx - 1; }\n"
-       "  else if (side == 0) { idx = (long)tid * g.P; stag = idx; nb = idx + 1; }\n"
-       "  else if (side == 2) { idx = (long)tid + (long)(g.H - 1) * g.P; stag = idx - g.P; nb = idx - g.P; }\n"
-       "  else { idx = (long)tid; stag = idx; nb = idx + g.P; }\n"
-       "  (void)stag;\n"
-       "  const bool xface = side <= 1;  // q = qx, q_lateral = qy on the left/right sides\n"
-       "  const float hnb = h[nb], bnb = sohle[nb], bidx = sohle[idx];\n"
-       "  const float qlnb = xface ? qy[nb] : qx[nb];\n"
-       "  const float qnb = xface ? qx[nb] : qy[nb];\n"
-       "  (void)hnb; (void)bnb; (void)bidx; (void)qlnb; (void)qnb;\n"
        "  const float qval = c->bc_q[which], zval = c->bc_z[which];\n"
-       "  (void)qval; (void)zval;\n"
        "  switch (which) {\n";
   for (int i = 0; i < n; i++) {
     if (lim[i] < bcs[i].from) continue;
-    const BCDesc& b = bcs[i];
-    const std::string Q = (b.side <= FIP_SIDE_RIGHT) ? "qx" : "qy";
-    const std::string QL = (b.side <= FIP_SIDE_RIGHT) ? "qy" : "qx";
-    const std::string qi = staggered ? "stag" : "idx";
-    // direction of the open-BC discharge: +1 on the right/top side, -1 on the left/bottom
-    const char* sign = (b.side == FIP_SIDE_RIGHT || b.side == FIP_SIDE_TOP) ? "1.0f" : "(-1.0f)";
-    std::string body;
-    if (b.type == FIP_BC_Z) {
-      if (staggered)
-        body = "h[idx] = fip_fmaxf(0.f, zval - bidx);\n"
-               "      h[nb] = fip_fmaxf(0.f, zval - bnb);\n"
-               "      " + Q + "[stag] = -0.f;\n";
-      else
-        body = "const float ah = fip_fmaxf(0.f, zval - bidx);\n"
-               "      h[idx] = ah;\n"
-               "      " + Q + "[idx] = 0.99f * (qnb / fip_fmaxf(hnb, 0.01f)) * ah;\n";
-    } else if (b.type == FIP_BC_Q) {
-      if (staggered)
-        body = "float ah = zval - bidx;\n      if (ah < 0.f) ah = 0.f;\n"
-               "      " + Q + "[stag] = __builtin_sqrtf(9.81f * ah) * ah * qval;\n"
-               "      if (ah > 0.f) { const float z_neighbor = hnb + bnb;\n"
-               "        h[idx] = fip_fmaxf(ah / 2, z_neighbor - bidx); }\n";
-      else
-        body = "float ah = zval - bidx;\n      if (ah < 0.f) ah = 0.f;\n"
-               "      " + Q + "[idx] = __builtin_sqrtf(9.81f * ah) * ah * qval;\n"
-               "      if (ah > 0.f) h[idx] = fip_fmaxf(ah / 2, hnb + bnb - bidx);\n";
-    } else if (b.type == FIP_BC_CLOSE) {
-      body = Q + "[" + qi + "] = 0.f;\n      " + QL + "[idx] = qlnb;\n      h[idx] = hnb;\n";
-    } else {
-      body = "const float ah_neighbor = hnb;\n      const float z_neighbor = ah_neighbor + bnb;\n"
-             "      if (ah_neighbor > 0.f) {\n"
-             "        " + Q + "[" + qi + "] = " + sign + " * __builtin_sqrtf(9.81f * ah_neighbor) * ah_neighbor;\n"
-             "        h[idx] = fip_fminf(ah_neighbor, fip_fmaxf(0.f, z_neighbor - bidx)); }\n";
-    }
-    std::snprintf(buf, sizeof buf, "  case %d: {\n      ", i);
+    std::snprintf(buf, sizeof buf,
+                  "  case %d: fip_bc_cell(%d, %d, %d, qval, zval, h, qx, qy, sohle, g.P, g.W, g.H, tid); break;\n",
+                  i, bcs[i].type, bcs[i].side, staggered ? 1 : 0);
     s += buf;
-    s += body;
-    s += "      break; }\n";
   }
   s += "  default: break;\n  }\n}\n";
   return s;
@@ -1095,3 +1032,57 @@ int fip_sync(fip_sim* s) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// library-internal helpers (fip_internal.h)
+
+namespace fip_internal {
+
+int fail(int code, const std::string& msg) { return ::fail(code, msg); }
+const fip_grid& grid(const fip_sim* s) { return s->g; }
+int staggered(const fip_sim* s) { return s->d.staggered; }
+int device(const fip_sim* s) { return s->dev; }
+
+int refresh(fip_sim* s) {
+  int it = 0;
+  int rc = ctl_read(s, CTL_OFF(iteration), &it);
+  if (rc) return rc;
+  s->host_iter = it;
+  s->started = true;
+  return FIP_OK;
+}
+
+int bc_stages(const fip_bc* bcs, int n, int W, int H, std::vector<int>& stage,
+              std::vector<int>& lim) {
+  // the reference launches one thread per position of the longer side (rounded up to
+  // 64), so positions beyond from + threads - 1 are never set (src/bc_kernel_gen.cpp:143-149)
+  const int ntid = (std::max(W, H) / 64 + 1) * 64;
+  stage.assign(n, -1);
+  lim.assign(n, 0);
+  for (int i = 0; i < n; i++) lim[i] = std::min(bcs[i].to, bcs[i].from + ntid - 1);
+  // the two cells deep strip a BC reads or writes at its positions
+  auto strip = [&](int i, int* x0, int* x1, int* y0, int* y1) {
+    const fip_bc& b = bcs[i];
+    if (b.side == FIP_SIDE_LEFT) { *x0 = 0; *x1 = 1; *y0 = b.from; *y1 = lim[i]; }
+    else if (b.side == FIP_SIDE_RIGHT) { *x0 = W - 2; *x1 = W - 1; *y0 = b.from; *y1 = lim[i]; }
+    else if (b.side == FIP_SIDE_TOP) { *x0 = b.from; *x1 = lim[i]; *y0 = H - 2; *y1 = H - 1; }
+    else { *x0 = b.from; *x1 = lim[i]; *y0 = 0; *y1 = 1; }
+  };
+  int nstages = 0;
+  for (int j = 0; j < n; j++) {
+    if (lim[j] < bcs[j].from) continue;
+    stage[j] = 0;
+    int a0, a1, b0, b1;
+    strip(j, &a0, &a1, &b0, &b1);
+    for (int i = 0; i < j; i++) {
+      if (stage[i] < 0) continue;
+      int c0, c1, d0, d1;
+      strip(i, &c0, &c1, &d0, &d1);
+      if (a0 <= c1 && c0 <= a1 && b0 <= d1 && d0 <= b1) stage[j] = std::max(stage[j], stage[i] + 1);
+    }
+    nstages = std::max(nstages, stage[j] + 1);
+  }
+  return nstages;
+}
+
+}  // namespace fip_internal

@@ -15,11 +15,15 @@ LIB_PATH = os.path.join(PKG, "libfiphip.so")
 
 FIELDS = {"h": 0, "qx": 1, "qy": 2, "hmax": 3, "bed": 4, "kst": 5}
 EV_NONE, EV_FINISHED, EV_DUMP = 0, 1, 2
-EXPORTS = ["fip_last_error", "fip_create", "fip_destroy", "fip_bc_generate", "fip_bc_launch",
+EXPORTS = ["fip_last_error", "fip_create", "fip_destroy", "fip_bc_generate", "fip_bc_source",
+           "fip_bc_launch",
            "fip_bc_apply", "fip_bc_params", "fip_control_timestep", "fip_compute_step",
            "fip_floodplain", "fip_set_run", "fip_simulate", "fip_read_samples", "fip_get_time",
            "fip_set_dt", "fip_download", "fip_upload", "fip_tiles", "fip_get_masks",
            "fip_get_cfl", "fip_set_timing", "fip_get_stats", "fip_sync"]
+ENS_EXPORTS = ["fip_ens_create", "fip_ens_destroy", "fip_ens_set_bcs", "fip_ens_set_run",
+               "fip_ens_simulate", "fip_ens_get_time", "fip_ens_download", "fip_ens_set_blocks",
+               "fip_ens_set_timing", "fip_ens_get_stats"]
 
 
 class FipDesc(C.Structure):
@@ -61,6 +65,8 @@ def lib():
         L.fip_create.argtypes = [C.POINTER(FipDesc), C.POINTER(vp)]
         L.fip_destroy.argtypes = [vp]
         L.fip_bc_generate.argtypes = [vp, C.POINTER(FipBC), C.c_int]
+        L.fip_bc_source.argtypes = [C.POINTER(FipBC), C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.fip_bc_launch.argtypes = [vp, C.POINTER(C.c_float)]
         L.fip_bc_apply.argtypes = [vp]
         L.fip_bc_params.argtypes = [vp, C.POINTER(C.c_float), C.c_int]
@@ -80,6 +86,16 @@ def lib():
         L.fip_get_cfl.argtypes = [vp, C.POINTER(C.c_float)]
         L.fip_set_timing.argtypes = [vp, C.c_int]
         L.fip_get_stats.argtypes = [vp, C.POINTER(FipStats)]
+        L.fip_ens_create.argtypes = [C.POINTER(FipDesc), C.c_int, C.POINTER(vp)]
+        L.fip_ens_destroy.argtypes = [vp]
+        L.fip_ens_set_bcs.argtypes = [vp, C.c_int, C.POINTER(FipBC), C.c_int]
+        L.fip_ens_set_run.argtypes = [vp, C.POINTER(C.c_double)]
+        L.fip_ens_simulate.argtypes = [vp, C.c_longlong, ip, C.POINTER(C.c_longlong)]
+        L.fip_ens_get_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_float), ip]
+        L.fip_ens_download.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        L.fip_ens_set_blocks.argtypes = [vp, C.c_int]
+        L.fip_ens_set_timing.argtypes = [vp, C.c_int]
+        L.fip_ens_get_stats.argtypes = [vp, C.POINTER(FipStats)]
         _lib = L
     return _lib
 
@@ -91,6 +107,105 @@ def _check(rc: int):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _desc(case: Case, keep: list, mode: str, variant: str, tile_rows: int, device: int,
+          staggered: bool) -> FipDesc:
+    def arr(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        keep.append(a)
+        return a
+
+    bed, h, qx, qy, kv = (arr(case.bed), arr(case.h), arr(case.qx), arr(case.qy),
+                          arr(case.kst_var))
+    return FipDesc(case.W, case.H, case.dx, case.dt,
+                   case.no_data if case.no_data is not None else -9999.0, case.kst,
+                   _ptr(bed), _ptr(h), _ptr(qx), _ptr(qy), _ptr(kv),
+                   1 if case.variable_dt else 0, 1 if staggered else 0,
+                   1 if mode == "masked" else 0, 1 if variant == "fast" else 0, tile_rows, device)
+
+
+def _bc_array(case: Case, bcs, keep: list):
+    arr = (FipBC * max(1, len(bcs)))()
+    for i, bc in enumerate(bcs):
+        frm, to = case.bc_range(bc)
+        b = arr[i]
+        b.type, b.side, b.from_, b.to = BC_TYPES[bc.type], BC_SIDES[bc.side], frm, to
+        b.z = bc.z if bc.z is not None else 0.0
+        b.q = bc.q if bc.q is not None else 0.0
+        for nm, ser in (("z", bc.z_series), ("q", bc.q_series)):
+            if ser is not None:
+                t = np.asarray(ser[0], dtype=np.float32)
+                v = np.asarray(ser[1], dtype=np.float32)
+                keep += [t, v]
+                setattr(b, nm + "t", _ptr(t))
+                setattr(b, nm + "v", _ptr(v))
+                setattr(b, nm + "n", len(t))
+    return arr
+
+
+class Ensemble:
+    """Many scenarios advanced together on one GPU (include/fip_ensemble.h)."""
+
+    def __init__(self, cases, mode: str = "masked", variant: str = "exact", tile_rows: int = 32,
+                 device: int = 0, staggered: bool = True):
+        L = lib()
+        self.cases = list(cases)
+        keep = []
+        descs = (FipDesc * len(self.cases))(*[_desc(c, keep, mode, variant, tile_rows, device,
+                                                     staggered) for c in self.cases])
+        e = C.c_void_p()
+        _check(L.fip_ens_create(descs, len(self.cases), C.byref(e)))
+        self.e = e
+        for i, c in enumerate(self.cases):
+            k2 = []
+            arr = _bc_array(c, c.bcs, k2)
+            _check(L.fip_ens_set_bcs(self.e, i, arr, len(c.bcs)))
+
+    def close(self):
+        if getattr(self, "e", None):
+            lib().fip_ens_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_run(self, targets):
+        t = (C.c_double * len(self.cases))(*[float(x) for x in targets])
+        _check(lib().fip_ens_set_run(self.e, t))
+
+    def simulate(self, nsteps):
+        act, done = C.c_int(), C.c_longlong()
+        _check(lib().fip_ens_simulate(self.e, int(nsteps), C.byref(act), C.byref(done)))
+        return act.value, done.value
+
+    def time(self, s):
+        t, dt, it = C.c_double(), C.c_float(), C.c_int()
+        _check(lib().fip_ens_get_time(self.e, s, C.byref(t), C.byref(dt), C.byref(it)))
+        return t.value, dt.value, it.value
+
+    def field(self, s, name, side=0):
+        c = self.cases[s]
+        a = np.empty((c.H, c.W), dtype=np.float32)
+        _check(lib().fip_ens_download(self.e, s, FIELDS[name], side,
+                                      a.ctypes.data_as(C.POINTER(C.c_float))))
+        return a
+
+    def set_blocks(self, blocks):
+        _check(lib().fip_ens_set_blocks(self.e, int(blocks)))
+
+    def set_timing(self, on: bool):
+        _check(lib().fip_ens_set_timing(self.e, 1 if on else 0))
+
+    def stats(self):
+        st = FipStats()
+        _check(lib().fip_ens_get_stats(self.e, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in FipStats._fields_}
 
 
 class Sim:

@@ -232,6 +232,40 @@ def config1(W: int = 5667, H: int = 6500, nhills: int = 2048, duration: float = 
                 kst_var=kst, no_data=-1.0, bcs=bcs)
 
 
+CONFIG2_SIZES = ((256, 256), (512, 384), (1024, 1024), (1536, 1024), (2048, 1536))
+
+
+def config2(scenario: int, sizes=CONFIG2_SIZES, duration: float = 6 * 3600.0) -> Case:
+    """One scenario of the batched ensemble: a config-0 style domain of a drawn size.
+
+    Size drawn uniformly from `sizes` (W, H), dx = 3 m; terrain as config 0 (slope 0.002,
+    64 hills per 512^2 of area, a 20-cell channel 3 m deep along y = H/2); uniform
+    kSt ~ U(20, 45); left type q over the channel with a triangular hydrograph rising
+    from 0 (60 s before t=0) to a peak ~ U(50, 800) m^3/s at t_peak ~ U(0.5, 3) h and
+    back to 0 at 6 h, z constant at the channel bed + 2.5 m; right open; top and bottom
+    closed.
+    """
+    rng = SplitMix64(seed_for(2, scenario))
+    W, H = sizes[min(len(sizes) - 1, int(rng.uniform(0, len(sizes))))]
+    dx = 3.0
+    nhills = max(1, int(round(64 * W * H / (512.0 * 512.0))))
+    bed = terrain_plane_hills(W, H, dx, 0.002, nhills, rng)
+    c = H // 2
+    bed[c - 10:c + 10, :] -= 3.0
+    kst = float(np.float32(rng.uniform(20.0, 45.0)))
+    peak = rng.uniform(50.0, 800.0)
+    tpeak = rng.uniform(0.5, 3.0) * 3600.0
+    bed32 = bed.astype(np.float32)
+    frm, to = c - 10, c + 9
+    zc = float(np.float32(bed32[frm:to + 1, 0].min()) + np.float32(2.5))
+    qt = [-60.0, tpeak, 6 * 3600.0]
+    qv = [0.0, peak, 0.0]
+    bcs = [BCSpec("q", "left", frm, to, z_series=([0.0, duration], [zc, zc]), q_series=(qt, qv)),
+           BCSpec("open", "right"), BCSpec("close", "top"), BCSpec("close", "bottom")]
+    return Case(name=f"cfg2_{scenario}_{W}x{H}", W=W, H=H, dx=dx, dt=0.1, duration=duration,
+                bed=bed32, kst=kst, bcs=bcs)
+
+
 def small_case(W: int, H: int, scenario: int = 0, wet: bool = True, kst_var: bool = False,
                no_data: bool = False, bc_kinds=("z", "open", "close", "close")) -> Case:
     """A small config-0 style case for parity tests, optionally with an initial lake."""

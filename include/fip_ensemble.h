@@ -1,0 +1,46 @@
+/* C ABI of libfiphip: ensembles of independent fip domains on one MI355X.
+ *
+ * An ensemble advances many flood scenarios (each its own grid size, terrain,
+ * roughness, boundary conditions, dt and target duration) together: every step is one
+ * persistent stencil launch over a global list of the active tiles of all scenarios,
+ * one post-step launch (one work-group per scenario: duration, dt control, BC
+ * parameters) and one launch per BC stage for all scenarios; the steps run as hipGraph
+ * chunks of 100. Each scenario computes exactly what a single-domain run of it computes
+ * (fip.h, same mode, variant and tile rows) without sampling and save_state.
+ *
+ * The reference runs one domain per process (src/main.cpp:204-263); this interface has
+ * no counterpart there. Return codes and fip_last_error() as in fip.h.
+ */
+#ifndef FIP_ENSEMBLE_H
+#define FIP_ENSEMBLE_H
+#include "fip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fip_ens fip_ens;
+
+/* n scenarios (1..1024), each as for fip_create. mode, variant, tile_rows and device are
+ * taken from descs[0] and apply to all; every scenario must have kst_var, or none. */
+int fip_ens_create(const fip_desc* descs, int n, fip_ens** out);
+int fip_ens_destroy(fip_ens* e);
+/* The BC list of scenario s (fip_bc as for fip_bc_generate). */
+int fip_ens_set_bcs(fip_ens* e, int s, const fip_bc* bcs, int n);
+/* Target duration of every scenario (n values); resets nothing else. */
+int fip_ens_set_run(fip_ens* e, const double* target_durations);
+/* Runs up to nsteps ensemble steps, or until every scenario reached its target.
+ * *active receives the number of scenarios still running. */
+int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps_done);
+int fip_ens_get_time(fip_ens* e, int s, double* duration, float* dt, int* iteration);
+int fip_ens_download(fip_ens* e, int s, int field, int side, float* dst);
+/* Persistent stencil blocks per launch (0 = default: 8 per CU). For the autotuner. */
+int fip_ens_set_blocks(fip_ens* e, int blocks);
+/* steps, tiles_total (all scenarios), tiles_computed (last step), stencil timing. */
+int fip_ens_set_timing(fip_ens* e, int enable);
+int fip_ens_get_stats(fip_ens* e, fip_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -6,7 +6,7 @@ from fip_amd.sim import FipBC, lib
 from fip_amd.workloads import BC_SIDES, BC_TYPES
 
 CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fip_amd", "csrc")
-HEADERS = ["fip_ctl.h", "fip_detmath.h", "fip_sched.h", "fip_step_common.h"]
+HEADERS = ["fip_ctl.h", "fip_detmath.h", "fip_sched.h", "fip_step_common.h", "fip_bc_common.h"]
 
 
 def bc_source(case, staggered=True):
