@@ -16,10 +16,10 @@
 
 hipError_t fip_launch_ens_stencil_exact(const fip_ens_scn* scn, const unsigned long long* list,
                                         fip_ens_ctl* ec, int par, unsigned cap, int tile_rows,
-                                        int kvar, int blocks, hipStream_t st);
+                                        int kvar, int blocks, int wpb, hipStream_t st);
 hipError_t fip_launch_ens_stencil_fast(const fip_ens_scn* scn, const unsigned long long* list,
                                        fip_ens_ctl* ec, int par, unsigned cap, int tile_rows,
-                                       int kvar, int blocks, hipStream_t st);
+                                       int kvar, int blocks, int wpb, hipStream_t st);
 hipError_t fip_launch_ens(int which, const fip_ens_scn* scn, int nscn, int total_tiles,
                           const fip_ens_bc* bcs, const float* series, const fip_ens_chunk* chunks,
                           int nchunks, fip_ens_ctl* ec, unsigned long long* list, int mode, int par,
@@ -63,6 +63,8 @@ struct fip_ens {
   std::vector<fip_ens_chunk*> d_chunks;  // per stage
   std::vector<int> nchunks;
   int total_tiles = 0;
+  int cus = 256;
+  int wpb = 4;
   int stencil_blocks = 0;
   int sched_blocks = 0;
   long long iter = 0;
@@ -175,9 +177,9 @@ int launch_stencil(fip_ens* e, int par) {
   if (e->timing) EHIP(hipEventRecord(e->ev0, e->stream));
   const unsigned cap = (unsigned)e->total_tiles;
   hipError_t r = e->fast ? fip_launch_ens_stencil_fast(e->d_scn, list, e->d_ctl, par, cap, e->R,
-                                                        e->kvar, e->stencil_blocks, e->stream)
+                                                        e->kvar, e->stencil_blocks, e->wpb, e->stream)
                          : fip_launch_ens_stencil_exact(e->d_scn, list, e->d_ctl, par, cap, e->R,
-                                                         e->kvar, e->stencil_blocks, e->stream);
+                                                         e->kvar, e->stencil_blocks, e->wpb, e->stream);
   if (r != hipSuccess) return fail(FIP_E_HIP, std::string("ensemble stencil launch: ") + hipGetErrorString(r));
   if (e->timing) {
     EHIP(hipEventRecord(e->ev1, e->stream));
@@ -291,6 +293,7 @@ int fip_ens_create(const fip_desc* descs, int n, fip_ens** out) {
   if (hipGetDeviceProperties(&prop, e->dev) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipGetDeviceProperties failed"));
   const int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  e->cus = cus;
   e->stencil_blocks = cus * 8;
   e->sched_blocks = cus;
   if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
@@ -357,16 +360,59 @@ int fip_ens_set_run(fip_ens* e, const double* targets) {
   return FIP_OK;
 }
 
-int fip_ens_set_blocks(fip_ens* e, int blocks) {
+int fip_ens_set_launch(fip_ens* e, int waves_per_block, int blocks_per_cu) {
   if (!e) return fail(FIP_E_INVALID, "null argument");
-  if (blocks < 0) return fail(FIP_E_INVALID, "blocks must be >= 0");
-  if (blocks == 0) {
-    hipDeviceProp_t prop;
-    EHIP(hipGetDeviceProperties(&prop, e->dev));
-    blocks = (prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256) * 8;
-  }
-  e->stencil_blocks = blocks;
+  if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu < 0 || blocks_per_cu > 64)
+    return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu 0..64");
+  e->wpb = waves_per_block == 0 ? 4 : waves_per_block;
+  e->stencil_blocks = e->cus * (blocks_per_cu == 0 ? 8 : blocks_per_cu);
   destroy_graphs(e);
+  return FIP_OK;
+}
+
+int fip_ens_tune(const fip_desc* descs, int n, const fip_bc* const* bcs, const int* nbcs,
+                 int steps, fip_tune_result* out) {
+  if (!descs || !out || !nbcs || (n > 0 && !bcs)) return fail(FIP_E_INVALID, "null argument");
+  if (steps < 1) return fail(FIP_E_INVALID, "steps must be at least 1");
+  static const int kRows[] = {16, 32, 64};
+  static const int kWaves[] = {1, 2, 4, 8};
+  static const int kBlocksPerCu[] = {4, 8, 16};
+  std::vector<fip_desc> dd(descs, descs + n);
+  fip_tune_result best{};
+  best.stencil_ms = 1e30;
+  std::vector<double> far(n, 1e300);
+  for (int R : kRows) {
+    for (auto& d : dd) d.tile_rows = R;
+    fip_ens* e = nullptr;
+    int rc = fip_ens_create(dd.data(), n, &e);
+    for (int s = 0; s < n && !rc; s++) rc = fip_ens_set_bcs(e, s, bcs[s], nbcs[s]);
+    if (!rc) rc = fip_ens_set_run(e, far.data());
+    for (int w : kWaves)
+      for (int bpc : kBlocksPerCu) {
+        if (rc) break;
+        int act = 0;
+        long long done = 0;
+        rc = fip_ens_set_launch(e, w, bpc);
+        if (!rc) rc = fip_ens_set_timing(e, 0);
+        if (!rc) rc = fip_ens_simulate(e, steps, &act, &done);
+        const fip_stats before = e ? e->stats : fip_stats{};
+        if (!rc) rc = fip_ens_set_timing(e, 1);
+        if (!rc) rc = fip_ens_simulate(e, steps, &act, &done);
+        if (rc) break;
+        const long long k = e->stats.timed_launches - before.timed_launches;
+        const double ms = k > 0 ? (e->stats.stencil_ms - before.stencil_ms) / (double)k : 1e30;
+        best.tried++;
+        if (ms < best.stencil_ms) {
+          best.stencil_ms = ms;
+          best.tile_rows = R;
+          best.waves_per_block = w;
+          best.blocks_per_cu = bpc;
+        }
+      }
+    if (e) fip_ens_destroy(e);
+    if (rc) return rc;
+  }
+  *out = best;
   return FIP_OK;
 }
 

@@ -16,9 +16,9 @@
 #include "fip_embed.h"  // generated: the device headers as strings for hipRTC
 #include "fip_internal.h"
 
-hipError_t fip_launch_stencil_exact(const fip_grid& g, int par, int masked, int blocks,
+hipError_t fip_launch_stencil_exact(const fip_grid& g, int par, int masked, int blocks, int wpb,
                                     hipStream_t st);
-hipError_t fip_launch_stencil_fast(const fip_grid& g, int par, int masked, int blocks,
+hipError_t fip_launch_stencil_fast(const fip_grid& g, int par, int masked, int blocks, int wpb,
                                    hipStream_t st);
 hipError_t fip_launch_misc(int which, const fip_grid& g, int arg, int blocks, hipStream_t st);
 
@@ -71,6 +71,9 @@ struct fip_sim {
   int host_iter = 0;
   bool masked = false;
   bool fast = false;
+  int cus = 256;           // compute units of the device
+  int wpb = 4;             // wavefronts per stencil work-group
+  int bpc = 8;             // persistent stencil work-groups per CU (masked mode)
   int stencil_blocks = 0;
   int sched_blocks = 0;
   // boundary kernel
@@ -97,11 +100,11 @@ struct fip_sim {
 namespace {
 
 int launch_stencil(fip_sim* s, int par) {
-  const int blocks = s->masked ? s->stencil_blocks : (s->ntiles + 3) / 4;
+  const int blocks = s->masked ? s->stencil_blocks : (s->ntiles + s->wpb - 1) / s->wpb;
   hipError_t e;
   if (s->timing) HIPTRY(hipEventRecord(s->ev0, s->stream));
-  e = s->fast ? fip_launch_stencil_fast(s->g, par, s->masked, blocks, s->stream)
-              : fip_launch_stencil_exact(s->g, par, s->masked, blocks, s->stream);
+  e = s->fast ? fip_launch_stencil_fast(s->g, par, s->masked, blocks, s->wpb, s->stream)
+              : fip_launch_stencil_exact(s->g, par, s->masked, blocks, s->wpb, s->stream);
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
   if (s->timing) {
     HIPTRY(hipEventRecord(s->ev1, s->stream));
@@ -615,8 +618,9 @@ int fip_create(const fip_desc* d, fip_sim** out) {
             hipMemcpy(g.ctl, hc, sizeof(fip_ctl), hipMemcpyHostToDevice) == hipSuccess;
   delete hc;
   if (!ok) return cleanup(fail(FIP_E_HIP, "control block allocation failed"));
-  int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-  s->stencil_blocks = std::max(1, std::min((s->ntiles + 3) / 4, cus * 8));
+  const int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  s->cus = cus;
+  s->stencil_blocks = std::max(1, std::min((s->ntiles + s->wpb - 1) / s->wpb, cus * s->bpc));
   s->sched_blocks = std::max(1, std::min((s->ntiles + 255) / 256, cus));
   if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipEventCreate failed"));
@@ -1010,6 +1014,64 @@ int fip_get_masks(fip_sim* s, uint64_t* dst, int n) {
 int fip_get_cfl(fip_sim* s, float* cfl) {
   if (!s || !cfl) return fail(FIP_E_INVALID, "null argument");
   return ctl_read(s, CTL_OFF(cfl_prev), cfl);
+}
+
+int fip_set_launch(fip_sim* s, int waves_per_block, int blocks_per_cu) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu < 0 || blocks_per_cu > 64)
+    return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu 0..64");
+  s->wpb = waves_per_block == 0 ? 4 : waves_per_block;
+  s->bpc = blocks_per_cu == 0 ? 8 : blocks_per_cu;
+  s->stencil_blocks = std::max(1, std::min((s->ntiles + s->wpb - 1) / s->wpb, s->cus * s->bpc));
+  destroy_graphs(s);
+  return FIP_OK;
+}
+
+int fip_tune(const fip_desc* d, const fip_bc* bcs, int nbc, int steps, fip_tune_result* out) {
+  if (!d || !out || (nbc > 0 && !bcs)) return fail(FIP_E_INVALID, "null argument");
+  if (steps < 1) return fail(FIP_E_INVALID, "steps must be at least 1");
+  static const int kRows[] = {16, 32, 64};
+  static const int kWaves[] = {1, 2, 4, 8};
+  static const int kBlocksPerCu[] = {4, 8, 16};
+  const bool masked = d->mode == FIP_MODE_MASKED;
+  fip_tune_result best{};
+  best.stencil_ms = 1e30;
+  for (int R : kRows) {
+    fip_desc dd = *d;
+    dd.tile_rows = R;
+    fip_sim* s = nullptr;
+    int rc = fip_create(&dd, &s);
+    if (!rc) rc = fip_bc_generate(s, bcs, nbc);
+    if (!rc) rc = fip_set_run(s, 1e300, 0, nullptr, nullptr, nullptr, 0, nullptr, 0);
+    for (int w : kWaves) {
+      for (int bpc : kBlocksPerCu) {
+        if (rc) break;
+        if (!masked && bpc != kBlocksPerCu[0]) continue;
+        int ev = 0;
+        long long done = 0;
+        rc = fip_set_launch(s, w, masked ? bpc : 0);
+        if (!rc) rc = fip_set_timing(s, 0);
+        if (!rc) rc = fip_simulate(s, steps, &ev, &done);
+        const fip_stats before = s ? s->stats : fip_stats{};
+        if (!rc) rc = fip_set_timing(s, 1);
+        if (!rc) rc = fip_simulate(s, steps, &ev, &done);
+        if (rc) break;
+        const long long n = s->stats.timed_launches - before.timed_launches;
+        const double ms = n > 0 ? (s->stats.stencil_ms - before.stencil_ms) / (double)n : 1e30;
+        best.tried++;
+        if (ms < best.stencil_ms) {
+          best.stencil_ms = ms;
+          best.tile_rows = R;
+          best.waves_per_block = w;
+          best.blocks_per_cu = masked ? bpc : 0;
+        }
+      }
+    }
+    if (s) fip_destroy(s);
+    if (rc) return rc;
+  }
+  *out = best;
+  return FIP_OK;
 }
 
 int fip_set_timing(fip_sim* s, int enable) {

@@ -20,9 +20,11 @@ EXPORTS = ["fip_last_error", "fip_create", "fip_destroy", "fip_bc_generate", "fi
            "fip_bc_apply", "fip_bc_params", "fip_control_timestep", "fip_compute_step",
            "fip_floodplain", "fip_set_run", "fip_simulate", "fip_read_samples", "fip_get_time",
            "fip_set_dt", "fip_download", "fip_upload", "fip_tiles", "fip_get_masks",
-           "fip_get_cfl", "fip_set_timing", "fip_get_stats", "fip_sync"]
+           "fip_get_cfl", "fip_set_launch", "fip_tune", "fip_set_timing", "fip_get_stats",
+           "fip_sync"]
 ENS_EXPORTS = ["fip_ens_create", "fip_ens_destroy", "fip_ens_set_bcs", "fip_ens_set_run",
-               "fip_ens_simulate", "fip_ens_get_time", "fip_ens_download", "fip_ens_set_blocks",
+               "fip_ens_simulate", "fip_ens_get_time", "fip_ens_download", "fip_ens_set_launch",
+               "fip_ens_tune",
                "fip_ens_set_timing", "fip_ens_get_stats"]
 
 
@@ -45,6 +47,11 @@ class FipStats(C.Structure):
     _fields_ = [("steps", C.c_longlong), ("tiles_total", C.c_longlong),
                 ("tiles_computed", C.c_longlong), ("stencil_launches", C.c_longlong),
                 ("stencil_ms", C.c_double), ("timed_launches", C.c_longlong)]
+
+
+class FipTune(C.Structure):
+    _fields_ = [("tile_rows", C.c_int), ("waves_per_block", C.c_int), ("blocks_per_cu", C.c_int),
+                ("stencil_ms", C.c_double), ("tried", C.c_int)]
 
 
 class FipError(RuntimeError):
@@ -93,7 +100,12 @@ def lib():
         L.fip_ens_simulate.argtypes = [vp, C.c_longlong, ip, C.POINTER(C.c_longlong)]
         L.fip_ens_get_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_float), ip]
         L.fip_ens_download.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
-        L.fip_ens_set_blocks.argtypes = [vp, C.c_int]
+        L.fip_ens_set_launch.argtypes = [vp, C.c_int, C.c_int]
+        L.fip_ens_tune.argtypes = [C.POINTER(FipDesc), C.c_int, C.POINTER(C.POINTER(FipBC)),
+                                   ip, C.c_int, C.POINTER(FipTune)]
+        L.fip_set_launch.argtypes = [vp, C.c_int, C.c_int]
+        L.fip_tune.argtypes = [C.POINTER(FipDesc), C.POINTER(FipBC), C.c_int, C.c_int,
+                               C.POINTER(FipTune)]
         L.fip_ens_set_timing.argtypes = [vp, C.c_int]
         L.fip_ens_get_stats.argtypes = [vp, C.POINTER(FipStats)]
         _lib = L
@@ -196,8 +208,8 @@ class Ensemble:
                                       a.ctypes.data_as(C.POINTER(C.c_float))))
         return a
 
-    def set_blocks(self, blocks):
-        _check(lib().fip_ens_set_blocks(self.e, int(blocks)))
+    def set_launch(self, waves_per_block=0, blocks_per_cu=0):
+        _check(lib().fip_ens_set_launch(self.e, int(waves_per_block), int(blocks_per_cu)))
 
     def set_timing(self, on: bool):
         _check(lib().fip_ens_set_timing(self.e, 1 if on else 0))
@@ -362,6 +374,9 @@ class Sim:
         _check(lib().fip_get_cfl(self.s, C.byref(c)))
         return c.value
 
+    def set_launch(self, waves_per_block=0, blocks_per_cu=0):
+        _check(lib().fip_set_launch(self.s, int(waves_per_block), int(blocks_per_cu)))
+
     def set_timing(self, on: bool):
         _check(lib().fip_set_timing(self.s, 1 if on else 0))
 
@@ -372,3 +387,31 @@ class Sim:
 
     def sync(self):
         _check(lib().fip_sync(self.s))
+
+
+def _tune_dict(r: FipTune) -> dict:
+    return {k: getattr(r, k) for k, _ in FipTune._fields_}
+
+
+def tune(case: Case, mode: str = "dense", variant: str = "fast", steps: int = 20,
+         staggered: bool = True) -> dict:
+    """fip_tune: the fastest tile rows / launch shape for this case."""
+    keep = []
+    d = _desc(case, keep, mode, variant, 0, 0, staggered)
+    arr = _bc_array(case, case.bcs, keep)
+    r = FipTune()
+    _check(lib().fip_tune(C.byref(d), arr, len(case.bcs), int(steps), C.byref(r)))
+    return _tune_dict(r)
+
+
+def tune_ensemble(cases, mode: str = "masked", variant: str = "fast", steps: int = 20,
+                  staggered: bool = True) -> dict:
+    """fip_ens_tune: the fastest tile rows / launch shape for an ensemble."""
+    keep = []
+    descs = (FipDesc * len(cases))(*[_desc(c, keep, mode, variant, 0, 0, staggered) for c in cases])
+    arrs = [_bc_array(c, c.bcs, keep) for c in cases]
+    ptrs = (C.POINTER(FipBC) * len(cases))(*[C.cast(a, C.POINTER(FipBC)) for a in arrs])
+    nbcs = (C.c_int * len(cases))(*[len(c.bcs) for c in cases])
+    r = FipTune()
+    _check(lib().fip_ens_tune(descs, len(cases), ptrs, nbcs, int(steps), C.byref(r)))
+    return _tune_dict(r)

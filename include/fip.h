@@ -160,6 +160,24 @@ int fip_upload(fip_sim* sim, int field, int side, const float* src);
 int fip_tiles(fip_sim* sim, int* tiles_x, int* tiles_y, int* tile_cols, int* tile_rows);
 int fip_get_masks(fip_sim* sim, uint64_t* dst, int n);
 int fip_get_cfl(fip_sim* sim, float* cfl_min);
+/* Launch shape of the stencil: wavefronts per work-group (1..16; 0 = 4) and, in masked
+ * mode, persistent work-groups per CU (1..64; 0 = 8). Results do not depend on it. */
+int fip_set_launch(fip_sim* sim, int waves_per_block, int blocks_per_cu);
+
+typedef struct fip_tune_result {
+  int tile_rows;         /* 16, 32 or 64 */
+  int waves_per_block;
+  int blocks_per_cu;     /* masked mode; 0 in dense mode */
+  double stencil_ms;     /* mean stencil launch time of the chosen shape */
+  int tried;             /* shapes timed */
+} fip_tune_result;
+
+/* Autotuner (no counterpart in the reference, whose tile shape is fixed at 28 x 50 by
+ * src/kernels.cu:441-455): times the stencil of this configuration for every tile-row
+ * count and launch shape over `steps` timed steps (after as many warm-up steps) from the
+ * initial state, and returns the fastest. In dense mode results do not depend on the
+ * choice; in masked mode the tile rows decide which tiles are skipped. */
+int fip_tune(const fip_desc* desc, const fip_bc* bcs, int nbc, int steps, fip_tune_result* out);
 int fip_set_timing(fip_sim* sim, int enable);
 int fip_get_stats(fip_sim* sim, fip_stats* out);
 int fip_sync(fip_sim* sim);

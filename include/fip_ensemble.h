@@ -34,8 +34,15 @@ int fip_ens_set_run(fip_ens* e, const double* target_durations);
 int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps_done);
 int fip_ens_get_time(fip_ens* e, int s, double* duration, float* dt, int* iteration);
 int fip_ens_download(fip_ens* e, int s, int field, int side, float* dst);
-/* Persistent stencil blocks per launch (0 = default: 8 per CU). For the autotuner. */
-int fip_ens_set_blocks(fip_ens* e, int blocks);
+/* Launch shape of the ensemble stencil: wavefronts per work-group (1..16; 0 = 4) and
+ * persistent work-groups per CU (1..64; 0 = 8). Results do not depend on it. */
+int fip_ens_set_launch(fip_ens* e, int waves_per_block, int blocks_per_cu);
+/* Autotuner of the ensemble (fip_tune for ensembles): times the ensemble stencil of
+ * these scenarios (bcs[s] holds nbcs[s] BCs of scenario s) for every tile-row count and
+ * launch shape over `steps` timed steps after as many warm-up steps, and returns the
+ * fastest. */
+int fip_ens_tune(const fip_desc* descs, int n, const fip_bc* const* bcs, const int* nbcs,
+                 int steps, fip_tune_result* out);
 /* steps, tiles_total (all scenarios), tiles_computed (last step), stencil timing. */
 int fip_ens_set_timing(fip_ens* e, int enable);
 int fip_ens_get_stats(fip_ens* e, fip_stats* out);
