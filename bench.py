@@ -10,7 +10,8 @@ the device-resident loop in hipGraph chunks. Inputs are resident in HBM before t
 Prints one JSON line: value = W*H*steps / wall seconds (Gcell-updates/s) in dense mode,
 the roofline of the stencil kernel (algorithmic 32 B per cell with kSt_var, timed with HIP
 events on the simulation's stream), the masked-mode effective rate, the exact-variant
-rate and the CPU oracle on a bounded sample.
+rate, the batched ensemble of 64 scenarios of uneven sizes (launch shape from the
+ensemble autotuner) and the CPU oracle on a bounded sample.
 """
 from __future__ import annotations
 
@@ -60,6 +61,47 @@ def run_rate(case, mode, variant, rows, steps, warmup, timed_steps):
     ms = (s1["stencil_ms"] - s0["stencil_ms"]) / max(1, s1["timed_launches"] - s0["timed_launches"])
     s.close()
     return (t1 - t0), ms, tiles
+
+
+def run_ensemble(variant, steps, warmup, nscn=64):
+    """Batched ensemble of config-2 scenarios (sizes 256^2 .. 2048x1536), each started from a
+    flat water level wetting 30 % of its cells; tile rows and launch shape from the ensemble
+    autotuner, per mode. Returns a dict for the JSON line."""
+    from fip_amd.sim import Ensemble, tune_ensemble
+    from fip_amd.workloads import config2
+    cases = [wet_state(config2(s)) for s in range(nscn)]
+    cells = sum(c.W * c.H for c in cases)
+    out = {"scenarios": nscn, "cells": cells}
+    for mode in ("dense", "masked"):
+        t0 = time.perf_counter()
+        best = tune_ensemble(cases, mode=mode, variant=variant, steps=10)
+        tune_s = time.perf_counter() - t0
+        e = Ensemble(cases, mode=mode, variant=variant, tile_rows=best["tile_rows"])
+        e.set_launch(best["waves_per_block"], best["blocks_per_cu"])
+        e.set_run([c.duration for c in cases])
+        e.simulate(warmup)
+        t_before = sum(e.time(k)[0] for k in range(nscn))
+        t0 = time.perf_counter()
+        act, done = e.simulate(steps)
+        wall = time.perf_counter() - t0
+        t_after = sum(e.time(k)[0] for k in range(nscn))
+        st = e.stats()
+        e.set_timing(True)
+        s0 = e.stats()
+        e.simulate(20)
+        s1 = e.stats()
+        e.close()
+        kms = (s1["stencil_ms"] - s0["stencil_ms"]) / max(1, s1["timed_launches"] - s0["timed_launches"])
+        out[mode] = {"gcells": round(cells * done / wall / 1e9, 3),
+                     "ms_per_step": round(wall / max(1, done) * 1e3, 4),
+                     "stencil_ms": round(kms, 4),
+                     "simulated_s_per_wall_s": round((t_after - t_before) / wall, 1),
+                     "tiles_active": st["tiles_computed"], "tiles_total": st["tiles_total"],
+                     "tuned": {k: best[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")},
+                     "tune_seconds": round(tune_s, 1)}
+        if mode == "dense":
+            out[mode]["roofline_frac"] = round(28 * cells / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
 
 
 def measured_traffic(variant, rows):
@@ -113,6 +155,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble leg")
+    ap.add_argument("--ensemble-steps", type=int, default=300)
     ap.add_argument("--calibrate", action="store_true",
                     help="run 1 GiB device copies (counter calibration) and exit")
     args = ap.parse_args()
@@ -172,6 +216,8 @@ def main():
         out["masked"] = {"gcells_effective": round(W * H * n3 / w3 / 1e9, 3),
                          "kernel_ms": round(ms3, 4), "tiles_active": tiles[0],
                          "tiles_total": tiles[1]}
+    if not args.no_extra and not args.no_ensemble:
+        out["ensemble"] = run_ensemble(args.variant, args.ensemble_steps, 50)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         rate, secs = cpu_baseline(case, args.cpu_steps, threads)

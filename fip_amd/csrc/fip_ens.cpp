@@ -16,10 +16,12 @@
 
 hipError_t fip_launch_ens_stencil_exact(const fip_ens_scn* scn, const unsigned long long* list,
                                         fip_ens_ctl* ec, int par, unsigned cap, int tile_rows,
-                                        int kvar, int blocks, int wpb, hipStream_t st);
+                                        int kvar, int blocks, int wpb, int persistent,
+                                        hipStream_t st);
 hipError_t fip_launch_ens_stencil_fast(const fip_ens_scn* scn, const unsigned long long* list,
                                        fip_ens_ctl* ec, int par, unsigned cap, int tile_rows,
-                                       int kvar, int blocks, int wpb, hipStream_t st);
+                                       int kvar, int blocks, int wpb, int persistent,
+                                       hipStream_t st);
 hipError_t fip_launch_ens(int which, const fip_ens_scn* scn, int nscn, int total_tiles,
                           const fip_ens_bc* bcs, const float* series, const fip_ens_chunk* chunks,
                           int nchunks, fip_ens_ctl* ec, unsigned long long* list, int mode, int par,
@@ -65,6 +67,7 @@ struct fip_ens {
   int total_tiles = 0;
   int cus = 256;
   int wpb = 4;
+  int bpc = 8;  // persistent work-groups per CU (0 = one wavefront per list entry)
   int stencil_blocks = 0;
   int sched_blocks = 0;
   long long iter = 0;
@@ -176,10 +179,12 @@ int launch_stencil(fip_ens* e, int par) {
   unsigned long long* list = e->d_list + (size_t)par * std::max(1, e->total_tiles);
   if (e->timing) EHIP(hipEventRecord(e->ev0, e->stream));
   const unsigned cap = (unsigned)e->total_tiles;
-  hipError_t r = e->fast ? fip_launch_ens_stencil_fast(e->d_scn, list, e->d_ctl, par, cap, e->R,
-                                                        e->kvar, e->stencil_blocks, e->wpb, e->stream)
-                         : fip_launch_ens_stencil_exact(e->d_scn, list, e->d_ctl, par, cap, e->R,
-                                                         e->kvar, e->stencil_blocks, e->wpb, e->stream);
+  const int persistent = e->bpc > 0 ? 1 : 0;
+  const int blocks = persistent ? e->cus * e->bpc : std::max(1, (e->total_tiles + e->wpb - 1) / e->wpb);
+  hipError_t r = e->fast ? fip_launch_ens_stencil_fast(e->d_scn, list, e->d_ctl, par, cap, e->R, e->kvar,
+                                                        blocks, e->wpb, persistent, e->stream)
+                         : fip_launch_ens_stencil_exact(e->d_scn, list, e->d_ctl, par, cap, e->R, e->kvar,
+                                                         blocks, e->wpb, persistent, e->stream);
   if (r != hipSuccess) return fail(FIP_E_HIP, std::string("ensemble stencil launch: ") + hipGetErrorString(r));
   if (e->timing) {
     EHIP(hipEventRecord(e->ev1, e->stream));
@@ -362,10 +367,10 @@ int fip_ens_set_run(fip_ens* e, const double* targets) {
 
 int fip_ens_set_launch(fip_ens* e, int waves_per_block, int blocks_per_cu) {
   if (!e) return fail(FIP_E_INVALID, "null argument");
-  if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu < 0 || blocks_per_cu > 64)
-    return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu 0..64");
+  if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu > 64)
+    return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu at most 64");
   e->wpb = waves_per_block == 0 ? 4 : waves_per_block;
-  e->stencil_blocks = e->cus * (blocks_per_cu == 0 ? 8 : blocks_per_cu);
+  e->bpc = blocks_per_cu < 0 ? 8 : blocks_per_cu;
   destroy_graphs(e);
   return FIP_OK;
 }
@@ -376,7 +381,7 @@ int fip_ens_tune(const fip_desc* descs, int n, const fip_bc* const* bcs, const i
   if (steps < 1) return fail(FIP_E_INVALID, "steps must be at least 1");
   static const int kRows[] = {16, 32, 64};
   static const int kWaves[] = {1, 2, 4, 8};
-  static const int kBlocksPerCu[] = {4, 8, 16};
+  static const int kBlocksPerCu[] = {0, 4, 8, 16};
   std::vector<fip_desc> dd(descs, descs + n);
   fip_tune_result best{};
   best.stencil_ms = 1e30;

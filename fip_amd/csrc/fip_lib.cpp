@@ -73,7 +73,7 @@ struct fip_sim {
   bool fast = false;
   int cus = 256;           // compute units of the device
   int wpb = 4;             // wavefronts per stencil work-group
-  int bpc = 8;             // persistent stencil work-groups per CU (masked mode)
+  int bpc = 8;             // masked mode: persistent work-groups per CU (0 = one wave per entry)
   int stencil_blocks = 0;
   int sched_blocks = 0;
   // boundary kernel
@@ -100,11 +100,12 @@ struct fip_sim {
 namespace {
 
 int launch_stencil(fip_sim* s, int par) {
-  const int blocks = s->masked ? s->stencil_blocks : (s->ntiles + s->wpb - 1) / s->wpb;
+  const int kind = s->masked ? (s->bpc > 0 ? 2 : 1) : 0;
+  const int blocks = kind == 2 ? s->stencil_blocks : (s->ntiles + s->wpb - 1) / s->wpb;
   hipError_t e;
   if (s->timing) HIPTRY(hipEventRecord(s->ev0, s->stream));
-  e = s->fast ? fip_launch_stencil_fast(s->g, par, s->masked, blocks, s->wpb, s->stream)
-              : fip_launch_stencil_exact(s->g, par, s->masked, blocks, s->wpb, s->stream);
+  e = s->fast ? fip_launch_stencil_fast(s->g, par, kind, blocks, s->wpb, s->stream)
+              : fip_launch_stencil_exact(s->g, par, kind, blocks, s->wpb, s->stream);
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
   if (s->timing) {
     HIPTRY(hipEventRecord(s->ev1, s->stream));
@@ -620,7 +621,7 @@ int fip_create(const fip_desc* d, fip_sim** out) {
   if (!ok) return cleanup(fail(FIP_E_HIP, "control block allocation failed"));
   const int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   s->cus = cus;
-  s->stencil_blocks = std::max(1, std::min((s->ntiles + s->wpb - 1) / s->wpb, cus * s->bpc));
+  s->stencil_blocks = std::max(1, cus * std::max(1, s->bpc));
   s->sched_blocks = std::max(1, std::min((s->ntiles + 255) / 256, cus));
   if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipEventCreate failed"));
@@ -1018,11 +1019,11 @@ int fip_get_cfl(fip_sim* s, float* cfl) {
 
 int fip_set_launch(fip_sim* s, int waves_per_block, int blocks_per_cu) {
   if (!s) return fail(FIP_E_INVALID, "null argument");
-  if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu < 0 || blocks_per_cu > 64)
-    return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu 0..64");
+  if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu > 64)
+    return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu at most 64");
   s->wpb = waves_per_block == 0 ? 4 : waves_per_block;
-  s->bpc = blocks_per_cu == 0 ? 8 : blocks_per_cu;
-  s->stencil_blocks = std::max(1, std::min((s->ntiles + s->wpb - 1) / s->wpb, s->cus * s->bpc));
+  s->bpc = blocks_per_cu < 0 ? 8 : blocks_per_cu;
+  s->stencil_blocks = std::max(1, s->cus * std::max(1, s->bpc));
   destroy_graphs(s);
   return FIP_OK;
 }
@@ -1032,7 +1033,7 @@ int fip_tune(const fip_desc* d, const fip_bc* bcs, int nbc, int steps, fip_tune_
   if (steps < 1) return fail(FIP_E_INVALID, "steps must be at least 1");
   static const int kRows[] = {16, 32, 64};
   static const int kWaves[] = {1, 2, 4, 8};
-  static const int kBlocksPerCu[] = {4, 8, 16};
+  static const int kBlocksPerCu[] = {0, 4, 8, 16};
   const bool masked = d->mode == FIP_MODE_MASKED;
   fip_tune_result best{};
   best.stencil_ms = 1e30;
@@ -1049,7 +1050,7 @@ int fip_tune(const fip_desc* d, const fip_bc* bcs, int nbc, int steps, fip_tune_
         if (!masked && bpc != kBlocksPerCu[0]) continue;
         int ev = 0;
         long long done = 0;
-        rc = fip_set_launch(s, w, masked ? bpc : 0);
+        rc = fip_set_launch(s, w, masked ? bpc : -1);
         if (!rc) rc = fip_set_timing(s, 0);
         if (!rc) rc = fip_simulate(s, steps, &ev, &done);
         const fip_stats before = s ? s->stats : fip_stats{};
@@ -1063,7 +1064,7 @@ int fip_tune(const fip_desc* d, const fip_bc* bcs, int nbc, int steps, fip_tune_
           best.stencil_ms = ms;
           best.tile_rows = R;
           best.waves_per_block = w;
-          best.blocks_per_cu = masked ? bpc : 0;
+          best.blocks_per_cu = masked ? bpc : -1;
         }
       }
     }

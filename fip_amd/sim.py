@@ -208,7 +208,7 @@ class Ensemble:
                                       a.ctypes.data_as(C.POINTER(C.c_float))))
         return a
 
-    def set_launch(self, waves_per_block=0, blocks_per_cu=0):
+    def set_launch(self, waves_per_block=0, blocks_per_cu=-1):
         _check(lib().fip_ens_set_launch(self.e, int(waves_per_block), int(blocks_per_cu)))
 
     def set_timing(self, on: bool):
@@ -374,7 +374,7 @@ class Sim:
         _check(lib().fip_get_cfl(self.s, C.byref(c)))
         return c.value
 
-    def set_launch(self, waves_per_block=0, blocks_per_cu=0):
+    def set_launch(self, waves_per_block=0, blocks_per_cu=-1):
         _check(lib().fip_set_launch(self.s, int(waves_per_block), int(blocks_per_cu)))
 
     def set_timing(self, on: bool):

@@ -161,13 +161,14 @@ int fip_tiles(fip_sim* sim, int* tiles_x, int* tiles_y, int* tile_cols, int* til
 int fip_get_masks(fip_sim* sim, uint64_t* dst, int n);
 int fip_get_cfl(fip_sim* sim, float* cfl_min);
 /* Launch shape of the stencil: wavefronts per work-group (1..16; 0 = 4) and, in masked
- * mode, persistent work-groups per CU (1..64; 0 = 8). Results do not depend on it. */
+ * mode, persistent work-groups per CU (1..64), or 0 for a grid of one wavefront per
+ * active-tile entry (< 0 = default 8). Results do not depend on it. */
 int fip_set_launch(fip_sim* sim, int waves_per_block, int blocks_per_cu);
 
 typedef struct fip_tune_result {
   int tile_rows;         /* 16, 32 or 64 */
   int waves_per_block;
-  int blocks_per_cu;     /* masked mode; 0 in dense mode */
+  int blocks_per_cu;     /* masked mode (0 = one wavefront per entry); -1 in dense mode */
   double stencil_ms;     /* mean stencil launch time of the chosen shape */
   int tried;             /* shapes timed */
 } fip_tune_result;

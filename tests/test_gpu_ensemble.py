@@ -102,6 +102,25 @@ def test_ensemble_fast_within_tolerance():
     e.close()
 
 
+def test_ensemble_launch_shapes_agree():
+    """Grid and persistent list kernels, any waves per work-group: same bits."""
+    from fip_amd.sim import Ensemble
+    cases = scenarios()[:4]
+    ref = None
+    for wpb, bpc in ((4, 8), (1, 0), (8, 0), (2, 16)):
+        e = Ensemble(cases, mode="masked", variant="exact", tile_rows=32)
+        e.set_launch(wpb, bpc)
+        e.set_run([c.duration for c in cases])
+        e.simulate(1_000_000)
+        got = [(e.time(k), e.field(k, "h"), e.field(k, "qx")) for k in range(len(cases))]
+        e.close()
+        if ref is None:
+            ref = got
+            continue
+        for (t1, h1, q1), (t2, h2, q2) in zip(ref, got):
+            assert t1 == t2 and same_bits(h1, h2) and same_bits(q1, q2), (wpb, bpc)
+
+
 def test_ensemble_chunked_runs_match_one_run():
     """Stopping and resuming (non-graph chunks, then graph chunks) changes nothing."""
     from fip_amd.sim import Ensemble
