@@ -31,8 +31,10 @@ UNITS = [
     ("stencil_fast.hip", ["-ffp-contract=fast", "-ffinite-math-only"]),
     ("kernels_misc.hip", ["-ffp-contract=off"]),
     ("ensemble.hip", ["-ffp-contract=off"]),
+    ("closure.hip", ["-ffp-contract=off"]),
     ("fip_lib.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("fip_ens.cpp", ["-ffp-contract=off", "-x", "hip"]),
+    ("closure_host.cpp", ["-ffp-contract=off", "-x", "hip"]),
 ]
 
 

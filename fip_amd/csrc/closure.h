@@ -1,0 +1,76 @@
+// Shared by the closure kernels (closure.hip) and their host code (closure_host.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fip_ctl.h"
+
+#define CL_C 32          // channels of every activation tensor
+#define CL_TAPS 9
+#define CL_FEAT 6        // real input features
+#define CL_OUT 2         // real outputs (corrections of qx and qy)
+#define CL_LAYERS 4
+#define CL_LAYER_PARAMS (CL_TAPS * CL_C * CL_C + CL_C)
+#define CL_PACKED (2 * CL_TAPS * 64 * 8)   // bf16 elements of one packed layer
+
+typedef unsigned short u16;
+enum { EPI_GELU = 0, EPI_OUT = 1, EPI_DGELU = 2 };
+
+// Input features of one coarse state: z - zref, qx, qy, bed - zref, subgrid bed std, kSt.
+struct ClFeat {
+  const float* h;
+  const float* qx;
+  const float* qy;
+  const float* bed;
+  const float* kst;    // NULL = uniform
+  const float* bstd;   // W x H dense
+  float kst_u, zref;
+  int W, H, P;
+};
+
+struct ClMass {
+  double v_prev;     // interior volume after the previous step
+  double phi;        // volume the held boundary faces bring in during the next step
+  double max_rel;    // largest relative mass error seen
+  double last_rel;
+  int valid;
+  int fallback;      // set once the relative mass error exceeded the limit
+  unsigned nsat;     // faces that kept the plain step because the network saturated
+  unsigned steps;
+};
+
+
+hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
+                          u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,
+                          hipStream_t st);
+hipError_t cl_mfma_probe_launch(const u16* A, const u16* B, float* D, hipStream_t st);
+hipError_t cl_restrict_launch(fip_grid f, int fcur, fip_grid c, int ccur, int F, hipStream_t st);
+hipError_t cl_restrict_flux_launch(fip_grid f, int fcur, int F, int Wc, int Hc, float* qt, uint8_t* mask,
+                                   hipStream_t st);
+hipError_t cl_flux_launch(fip_grid g, int cur, float* q, hipStream_t st);
+hipError_t cl_features_launch(ClFeat s, u16* out, int cstride, const fip_ctl* ctl, int par, hipStream_t st);
+hipError_t cl_features_ens_launch(const fip_ens_scn* scn, const float* bstd, const float* zref, int par,
+                                  u16* out, int W, int H, int n, hipStream_t st);
+hipError_t cl_gather_launch(const u16* samples, long sample_elems, int W, const int* origin, int ps, int ph,
+                            u16* out, int nb, hipStream_t st);
+hipError_t cl_gather_targets_launch(const float* qp, const float* qt, const uint8_t* mask, long sample_pixels,
+                                    int W, const int* origin, int ps, int ph, float* bqp, float* bqt,
+                                    uint8_t* bmask, int nb, hipStream_t st);
+hipError_t cl_feat_from_float_launch(const float* f, long npix, u16* out, hipStream_t st);
+hipError_t cl_wgrad_launch(const u16* X, const u16* dZ, float* part, int N, int H, int W, int nblocks,
+                           hipStream_t st);
+hipError_t cl_reduce_launch(const float* part, int nparts, int n, float* out, hipStream_t st);
+hipError_t cl_loss_launch(const float* o, const float* qp, const float* qt, const uint8_t* mask, long npix,
+                          float bound, float inv_count, u16* dz, double* part, hipStream_t st);
+hipError_t cl_pack_launch(const float* w, int nlayers, u16* wf, u16* wb, hipStream_t st);
+hipError_t cl_adam_launch(float* w, float* m, float* v, const float* g, int nlayers, float lr, float b1,
+                          float b2, float eps, float c1, float c2, hipStream_t st);
+hipError_t cl_correct_launch(fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms,
+                             hipStream_t st);
+hipError_t cl_mass_sum_launch(fip_grid g, int par, double* acc, hipStream_t st);
+hipError_t cl_mass_check_launch(fip_grid g, int par, double* acc, ClMass* ms, double limit, hipStream_t st);
+hipError_t cl_correct_ens_launch(const fip_ens_scn* scn, int par, const float* o, float bound, float sat,
+                                 ClMass* ms, int W, int H, int n, hipStream_t st);
+hipError_t cl_mass_sum_ens_launch(const fip_ens_scn* scn, int par, double* acc, int W, int H, int n, hipStream_t st);
+hipError_t cl_mass_check_ens_launch(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
+                                    hipStream_t st);

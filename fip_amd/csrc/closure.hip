@@ -1,0 +1,720 @@
+// Learned subgrid closure on gfx950: 4x restriction of a fine domain onto a coarse one,
+// the CNN that corrects coarse face discharges (4 layers of 3x3 convolutions, 32
+// channels, GELU, bounded tanh output), its backward pass and Adam.
+//
+// Convolutions are implicit GEMMs on the bf16 matrix cores (v_mfma_f32_16x16x32_bf16,
+// fp32 accumulation). Activations are NHWC bf16 with 32 channels (the 6 input features
+// and the 2 outputs are zero-padded to 32, so every layer is one kernel shape). For
+// D[co][pixel] = sum_k A[co][k] B[k][pixel] with k = (tap, ci):
+//   A fragment: lane l holds A[l % 16][8 (l / 16) + j], j = 0..7 (packed weights)
+//   B fragment: lane l holds B[8 (l / 16) + j][l % 16]    (8 channels of one pixel)
+//   D:          lane l holds D[4 (l / 16) + r][l % 16], r = 0..3 (4 channels of a pixel)
+// The closure has no counterpart in the reference.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "closure.h"
+#include "fip_ctl.h"
+#include "fip_detmath.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+// float -> bf16, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ u16 f2bf(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
+
+// GELU, tanh form: 0.5 x (1 + tanh(s (x + 0.044715 x^3))), s = sqrt(2/pi)
+__device__ __forceinline__ float gelu(float x) {
+  const float s = 0.7978845608028654f;
+  return 0.5f * x * (1.f + tanhf(s * (x + 0.044715f * x * x * x)));
+}
+__device__ __forceinline__ float dgelu(float x) {
+  const float s = 0.7978845608028654f;
+  const float u = s * (x + 0.044715f * x * x * x);
+  const float t = tanhf(u);
+  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * s * (1.f + 3.f * 0.044715f * x * x);
+}
+
+__device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 ld8(const u16* p) {  // 16-byte aligned
+  const uint4 v = *(const uint4*)p;
+  return __builtin_bit_cast(bf16x8, v);
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Layout check of the matrix instruction: D = A B for 16x16x32 operands given row-major
+// in A (16 x 32) and B (32 x 16); one wavefront. Used by a test before the convolutions.
+extern "C" __global__ void cl_mfma_probe(const u16* A, const u16* B, float* D) {
+  const int l = threadIdx.x;
+  __attribute__((aligned(16))) u16 a[8], b[8];
+  for (int j = 0; j < 8; j++) {
+    a[j] = A[(l % 16) * 32 + 8 * (l / 16) + j];
+    b[j] = B[(8 * (l / 16) + j) * 16 + (l % 16)];
+  }
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = mfma(ld8(a), ld8(b), acc);
+  for (int r = 0; r < 4; r++) D[(4 * (l / 16) + r) * 16 + (l % 16)] = acc[r];
+}
+
+// ---------------------------------------------------------------------------------------
+// Restriction by a factor F of a fine domain's current state onto a coarse domain (both
+// pitched fip_grid layouts): h = block mean of h (so the coarse volume is the fine one);
+// qx on the coarse east face = mean of the F fine east faces it covers (total discharge
+// through the face is kept), qy likewise on the north face. Both ping-pong sides of the
+// coarse domain receive the state; the coarse duration becomes the fine one.
+extern "C" __global__ void cl_restrict(fip_grid f, int fcur, fip_grid c, int ccur, int F) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && J == 0) c.ctl->duration = f.ctl->duration;
+  if (I >= c.W || J >= c.H) return;
+  double sh = 0.0, sx = 0.0, sy = 0.0;
+  for (int r = 0; r < F; r++)
+    for (int q = 0; q < F; q++) sh += (double)f.h[fcur][(long)(F * J + r) * f.P + F * I + q];
+  for (int r = 0; r < F; r++) sx += (double)f.qx[fcur][(long)(F * J + r) * f.P + F * I + F - 1];
+  for (int q = 0; q < F; q++) sy += (double)f.qy[fcur][(long)(F * J + F - 1) * f.P + F * I + q];
+  const double inv = 1.0 / F;
+  const long o = (long)J * c.P + I;
+  const float h = (float)(sh * inv * inv), qx = (float)(sx * inv), qy = (float)(sy * inv);
+  c.h[ccur][o] = h; c.h[ccur ^ 1][o] = h;
+  c.qx[ccur][o] = qx; c.qx[ccur ^ 1][o] = qx;
+  c.qy[ccur][o] = qy; c.qy[ccur ^ 1][o] = qy;
+}
+
+// ---------------------------------------------------------------------------------------
+// Input features of one coarse state into NHWC bf16 with `cstride` channels (8 for stored
+// samples, 32 for a network input): z - zref, qx, qy, bed - zref, subgrid bed std, kSt,
+// each scaled to order one. skip_if_halted: do nothing when the step was skipped.
+__device__ __forceinline__ void feat_pixel(const ClFeat& s, int x, int y, float v[CL_FEAT]) {
+  const long o = (long)y * s.P + x;
+  const float b = s.bed[o];
+  v[0] = (s.h[o] + b - s.zref) * 0.25f;
+  v[1] = s.qx[o];
+  v[2] = s.qy[o];
+  v[3] = (b - s.zref) * 0.25f;
+  v[4] = s.bstd[(long)y * s.W + x] * 2.0f;
+  v[5] = (s.kst ? s.kst[o] : s.kst_u) * (1.0f / 30.0f);
+}
+
+extern "C" __global__ void cl_features(ClFeat s, u16* out, int cstride, const fip_ctl* ctl, int par) {
+  if (ctl && ctl->halt_req[par ^ 1]) return;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= s.W || y >= s.H) return;
+  float v[CL_FEAT];
+  feat_pixel(s, x, y, v);
+  u16* p = out + ((long)y * s.W + x) * cstride;
+  for (int k = 0; k < cstride; k++) p[k] = k < CL_FEAT ? f2bf(v[k]) : (u16)0;
+}
+
+// Patches of stored samples (NHWC, 8 channels) into a batch (NHWC, 32 channels): patch b
+// is ps wide and gridDim.y high, origin[3 b .. 3 b + 2] = (sample, x0, y0).
+extern "C" __global__ void cl_gather(const u16* samples, long sample_elems, int W, const int* origin,
+                                     int ps, u16* out) {
+  const int b = blockIdx.z, y = blockIdx.y, x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= ps) return;
+  const int s = origin[3 * b], x0 = origin[3 * b + 1], y0 = origin[3 * b + 2];
+  const u16* src = samples + s * sample_elems + ((long)(y0 + y) * W + (x0 + x)) * 8;
+  const uint4 v = *(const uint4*)src;
+  uint4* dst = (uint4*)(out + (((long)b * gridDim.y + y) * ps + x) * CL_C);
+  dst[0] = v;
+  dst[1] = make_uint4(0, 0, 0, 0);
+  dst[2] = make_uint4(0, 0, 0, 0);
+  dst[3] = make_uint4(0, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------------------
+// 3x3 convolution (padding 1) of N images H x W, NHWC bf16 with 32 channels, as an
+// implicit GEMM. One wavefront computes 64 pixels of one row x 32 output channels; a
+// work-group holds 4 wavefronts on 4 consecutive rows. Epilogues:
+//   EPI_GELU : Z = acc + bias (bf16), A = GELU(Z) (bf16)
+//   EPI_OUT  : out = acc + bias, channels 0..1 as fp32 [pixel][2]
+//   EPI_DGELU: out = acc * GELU'(Zp) (bf16)  (backward through the previous layer)
+// wp: packed weights, [mt 0..1][tap 0..8][lane 0..63][8] bf16 (A fragments).
+
+template <int EPI>
+__global__ __launch_bounds__(256) void cl_conv(const u16* __restrict__ X, const u16* __restrict__ wp,
+                                               const float* __restrict__ bias, const u16* __restrict__ Zp,
+                                               u16* __restrict__ Z, u16* __restrict__ A,
+                                               float* __restrict__ out, int N, int H, int W,
+                                               const fip_ctl* ctl, int par) {
+  if (ctl && ctl->halt_req[par ^ 1]) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = blockIdx.z, y = blockIdx.y * 4 + wave, x0 = blockIdx.x * 64;
+  if (y >= H || n >= N) return;
+  const int col = lane & 15, grp = lane >> 4;
+  f32x4 acc[2][4];
+  for (int m = 0; m < 2; m++)
+    for (int t = 0; t < 4; t++) acc[m][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const u16* img = X + (long)n * H * W * CL_C;
+#pragma unroll 1
+  for (int tap = 0; tap < CL_TAPS; tap++) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int yy = y + dy;
+    const bool rowok = yy >= 0 && yy < H;
+    bf16x8 b[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int xx = x0 + t * 16 + col + dx;
+      b[t] = (rowok && xx >= 0 && xx < W) ? ld8(img + ((long)yy * W + xx) * CL_C + 8 * grp) : zero8();
+    }
+    const bf16x8 a0 = ld8(wp + ((0 * CL_TAPS + tap) * 64 + lane) * 8);
+    const bf16x8 a1 = ld8(wp + ((1 * CL_TAPS + tap) * 64 + lane) * 8);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      acc[0][t] = mfma(a0, b[t], acc[0][t]);
+      acc[1][t] = mfma(a1, b[t], acc[1][t]);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int x = x0 + t * 16 + col;
+      if (x >= W) continue;
+      const long pix = ((long)n * H + y) * W + x;
+      const int co0 = m * 16 + 4 * grp;
+      if constexpr (EPI == EPI_GELU) {
+        __attribute__((aligned(8))) u16 zb[4], ab[4];
+        for (int r = 0; r < 4; r++) {
+          const float z = acc[m][t][r] + bias[co0 + r];
+          const u16 zq = f2bf(z);
+          zb[r] = zq;
+          ab[r] = f2bf(gelu(bf2f(zq)));
+        }
+        *(uint2*)(Z + pix * CL_C + co0) = *(const uint2*)zb;
+        *(uint2*)(A + pix * CL_C + co0) = *(const uint2*)ab;
+      } else if constexpr (EPI == EPI_OUT) {
+        if (co0 == 0) {
+          out[pix * 2 + 0] = acc[m][t][0] + bias[0];
+          out[pix * 2 + 1] = acc[m][t][1] + bias[1];
+        }
+      } else {
+        const uint2 zr = *(const uint2*)(Zp + pix * CL_C + co0);
+        const u16* zp = (const u16*)&zr;
+        __attribute__((aligned(8))) u16 ob[4];
+        for (int r = 0; r < 4; r++) ob[r] = f2bf(acc[m][t][r] * dgelu(bf2f(zp[r])));
+        *(uint2*)(Z + pix * CL_C + co0) = *(const uint2*)ob;
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Weight gradient of one conv layer: dW[tap][ci][co] = sum_p X[p + off(tap)][ci] dZ[p][co]
+// and db[co] = sum_p dZ[p][co], as GEMMs over pixels on the matrix cores. A work-group
+// takes 4-row x 64-pixel blocks (grid-stride), stages X (with a one-pixel halo) and dZ
+// transposed to [channel][pixel] in LDS, and wavefront w accumulates taps w, w+4, w+8.
+// Partial sums go to part[block][9*32*32 + 32] and are added by cl_reduce.
+#define WG_ROWS 4
+#define WG_PX 64
+#define XS_W (WG_PX + 2)
+#define XS_H (WG_ROWS + 2)
+extern "C" __global__ __launch_bounds__(256) void cl_wgrad(const u16* __restrict__ X,
+                                                          const u16* __restrict__ dZ,
+                                                          float* __restrict__ part, int N, int H, int W) {
+  __shared__ u16 xs[CL_C][XS_H][XS_W + 2];
+  __shared__ __attribute__((aligned(16))) u16 zs[CL_C][WG_ROWS * WG_PX];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col = lane & 15, grp = lane >> 4;
+  const int segs = (W + WG_PX - 1) / WG_PX, bands = (H + WG_ROWS - 1) / WG_ROWS;
+  const long nblk = (long)N * bands * segs;
+  // accumulators: taps wave, wave+4, wave+8 (up to 3) x co tiles 2 x ci tiles 2
+  f32x4 acc[3][2][2];
+  for (int a = 0; a < 3; a++)
+    for (int m = 0; m < 2; m++)
+      for (int c = 0; c < 2; c++) acc[a][m][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  float dbias = 0.f;
+  for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int seg = (int)(blk % segs), band = (int)((blk / segs) % bands), n = (int)(blk / ((long)segs * bands));
+    const int y0 = band * WG_ROWS, x0 = seg * WG_PX;
+    __syncthreads();
+    // stage X rows y0-1 .. y0+WG_ROWS, pixels x0-1 .. x0+WG_PX (zero outside the image)
+    for (int e = threadIdx.x; e < XS_H * XS_W; e += blockDim.x) {
+      const int r = e / XS_W, q = e % XS_W;
+      const int yy = y0 - 1 + r, xx = x0 - 1 + q;
+      const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+      const u16* src = X + (((long)n * H + yy) * W + xx) * CL_C;
+      for (int c = 0; c < CL_C; c += 8) {
+        uint4 v = ok ? *(const uint4*)(src + c) : make_uint4(0, 0, 0, 0);
+        const u16* vv = (const u16*)&v;
+        for (int j = 0; j < 8; j++) xs[c + j][r][q] = vv[j];
+      }
+    }
+    for (int e = threadIdx.x; e < WG_ROWS * WG_PX; e += blockDim.x) {
+      const int r = e / WG_PX, q = e % WG_PX;
+      const int yy = y0 + r, xx = x0 + q;
+      const bool ok = yy < H && xx < W;
+      const u16* src = dZ + (((long)n * H + yy) * W + xx) * CL_C;
+      for (int c = 0; c < CL_C; c += 8) {
+        uint4 v = ok ? *(const uint4*)(src + c) : make_uint4(0, 0, 0, 0);
+        const u16* vv = (const u16*)&v;
+        for (int j = 0; j < 8; j++) zs[c + j][e] = vv[j];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < CL_C) {
+      float s = 0.f;
+      for (int e = 0; e < WG_ROWS * WG_PX; e++) s += bf2f(zs[threadIdx.x][e]);
+      dbias += s;
+    }
+    // K loop: 32 pixels per chunk (half a row)
+    for (int kc = 0; kc < WG_ROWS * WG_PX / 32; kc++) {
+      const int r = kc / (WG_PX / 32), px0 = (kc % (WG_PX / 32)) * 32;
+      bf16x8 za[2];
+      for (int m = 0; m < 2; m++) za[m] = ld8(&zs[m * 16 + col][r * WG_PX + px0 + 8 * grp]);
+      for (int a = 0; a < 3; a++) {
+        const int tap = wave + 4 * a;
+        if (tap >= CL_TAPS) break;
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        for (int c = 0; c < 2; c++) {
+          __attribute__((aligned(16))) u16 xb[8];
+          const u16* src = &xs[c * 16 + col][r + 1 + dy][px0 + 8 * grp + 1 + dx];
+          for (int j = 0; j < 8; j++) xb[j] = src[j];
+          const bf16x8 xv = ld8(xb);
+          for (int m = 0; m < 2; m++) acc[a][m][c] = mfma(za[m], xv, acc[a][m][c]);
+        }
+      }
+    }
+  }
+  // D[co][ci] of tile (m, c): lane holds co = m*16 + 4 grp + rr, ci = c*16 + col
+  float* pb = part + (long)blockIdx.x * (CL_TAPS * CL_C * CL_C + CL_C);
+  for (int a = 0; a < 3; a++) {
+    const int tap = wave + 4 * a;
+    if (tap >= CL_TAPS) break;
+    for (int m = 0; m < 2; m++)
+      for (int c = 0; c < 2; c++)
+        for (int rr = 0; rr < 4; rr++) {
+          const int co = m * 16 + 4 * grp + rr, ci = c * 16 + col;
+          pb[(tap * CL_C + ci) * CL_C + co] = acc[a][m][c][rr];
+        }
+  }
+  if (threadIdx.x < CL_C) pb[CL_TAPS * CL_C * CL_C + threadIdx.x] = dbias;
+}
+
+// out[i] = sum_b part[b][i] (fixed order: deterministic)
+extern "C" __global__ void cl_reduce(const float* part, int nparts, int n, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int b = 0; b < nparts; b++) s += part[(long)b * n + i];
+  out[i] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Loss of a batch and its gradient. Per pixel and output channel k (qx, qy):
+//   c = tanh(o) * bound * |qp|,  r = qp + c - qt,  loss = sum m r^2 / sum m
+// with m = 1 on trainable faces (mask != 0). Writes dZ4 (bf16, channels 0..1, others 0)
+// = d loss / d o scaled by gscale, and per-block partial sums of (m r^2, m, m (qp - qt)^2).
+extern "C" __global__ void cl_loss(const float* o, const float* qp, const float* qt, const uint8_t* mask,
+                                   long npix, float bound, float inv_count, u16* dz, double* part) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double s_r = 0.0, s_m = 0.0, s_p = 0.0;
+  if (p < npix) {
+    __attribute__((aligned(16))) u16 g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 2; k++) {
+      const float m = mask[p * 2 + k] ? 1.f : 0.f;
+      const float q = qp[p * 2 + k], tq = qt[p * 2 + k];
+      const float th = tanhf(o[p * 2 + k]);
+      const float sc = bound * fabsf(q);
+      const float r = q + th * sc - tq;
+      s_r += (double)(m * r * r);
+      s_m += (double)m;
+      s_p += (double)(m * (q - tq) * (q - tq));
+      g[k] = f2bf(m * 2.f * r * inv_count * sc * (1.f - th * th));
+    }
+    uint4* d = (uint4*)(dz + p * CL_C);
+    d[0] = *(const uint4*)g;
+    d[1] = make_uint4(0, 0, 0, 0);
+    d[2] = make_uint4(0, 0, 0, 0);
+    d[3] = make_uint4(0, 0, 0, 0);
+  }
+  __shared__ double sh[3][256];
+  sh[0][threadIdx.x] = s_r;
+  sh[1][threadIdx.x] = s_m;
+  sh[2][threadIdx.x] = s_p;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; k++) part[(long)blockIdx.x * 3 + k] = sh[k][0];
+}
+
+// ---------------------------------------------------------------------------------------
+// Adam on the fp32 master parameters (all layers, padded layout [layer][9*32*32 + 32]),
+// keeping padded entries at 0, then the packed bf16 weights of the forward pass and of the
+// backward-data pass (taps mirrored, ci and co swapped).
+//   m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; w -= lr (m / (1-b1^t)) / (sqrt(v / (1-b2^t)) + eps)
+__device__ __forceinline__ bool cl_param_live(int layer, int idx, int nlayers) {
+  if (idx >= CL_TAPS * CL_C * CL_C) {  // bias
+    const int co = idx - CL_TAPS * CL_C * CL_C;
+    return layer < nlayers - 1 || co < CL_OUT;
+  }
+  const int ci = (idx / CL_C) % CL_C, co = idx % CL_C;
+  if (layer == 0 && ci >= CL_FEAT) return false;
+  if (layer == nlayers - 1 && co >= CL_OUT) return false;
+  return true;
+}
+
+// packed index of W[tap][ci][co] in the A-fragment layout [mt][tap][lane][8]
+__device__ __forceinline__ long cl_pack_index(int tap, int ci, int co) {
+  const int mt = co / 16, lane = (co % 16) + 16 * (ci / 8), j = ci % 8;
+  return (((long)mt * CL_TAPS + tap) * 64 + lane) * 8 + j;
+}
+
+extern "C" __global__ void cl_pack(const float* w, int nlayers, u16* wf, u16* wb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nlayers * CL_LAYER_PARAMS) return;
+  const int layer = i / CL_LAYER_PARAMS, idx = i % CL_LAYER_PARAMS;
+  if (idx >= CL_TAPS * CL_C * CL_C) return;
+  const int tap = idx / (CL_C * CL_C), ci = (idx / CL_C) % CL_C, co = idx % CL_C;
+  const u16 q = f2bf(w[i]);
+  const long lw = (long)layer * 2 * CL_TAPS * 64 * 8;
+  wf[lw + cl_pack_index(tap, ci, co)] = q;
+  wb[lw + cl_pack_index(CL_TAPS - 1 - tap, co, ci)] = q;
+}
+
+extern "C" __global__ void cl_adam(float* w, float* m, float* v, const float* g, int nlayers, float lr,
+                                   float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nlayers * CL_LAYER_PARAMS) return;
+  const int layer = i / CL_LAYER_PARAMS, idx = i % CL_LAYER_PARAMS;
+  if (!cl_param_live(layer, idx, nlayers)) {
+    w[i] = 0.f;
+    return;
+  }
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------------------
+// Correction of a coarse domain's current face discharges by the network output o (fp32
+// [pixel][2]): on interior faces (qx: columns 1..W-3, rows 2..H-3; qy: columns 2..W-3,
+// rows 1..H-3, the faces the stencil updates; the held boundary faces are never touched)
+// q += tanh(o) * bound * |q|, unless |o| exceeds `sat` (the network saturates: that face
+// keeps the plain step) or the domain's fallback flag is set. A corrected face moves
+// water between two cells of the domain, so the volume is conserved by construction.
+extern "C" __global__ void cl_correct(fip_grid g, int par, const float* o, float bound, float sat,
+                                      ClMass* ms) {
+  if (g.ctl->halt_req[par ^ 1] || ms->fallback) return;
+  const int cur = par ^ 1;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  unsigned nsat = 0;
+  if (x < g.W && y < g.H) {
+    const long p = (long)y * g.W + x, i = (long)y * g.P + x;
+    if (x >= 1 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) {
+      const float ox = o[p * 2];
+      const float q = g.qx[cur][i];
+      if (fabsf(ox) <= sat) g.qx[cur][i] = q + tanhf(ox) * bound * fabsf(q);
+      else nsat++;
+    }
+    if (x >= 2 && x <= g.W - 3 && y >= 1 && y <= g.H - 3) {
+      const float oy = o[p * 2 + 1];
+      const float q = g.qy[cur][i];
+      if (fabsf(oy) <= sat) g.qy[cur][i] = q + tanhf(oy) * bound * fabsf(q);
+      else nsat++;
+    }
+  }
+  if (nsat) atomicAdd(&ms->nsat, nsat);
+}
+
+// Mass balance of a coarse domain without no_data cells, after a step and its
+// correction. The continuity update of the stored cells [1, W-2] x [1, H-2] telescopes, so
+// their volume changes in the next step by exactly what the held faces (qx at columns 0
+// and W-2, qy at rows 0 and H-2) bring in: phi = -dt dx (sum_j qx(W-2,j) - qx(0,j) +
+// sum_i qy(i,H-2) - qy(i,0)). cl_mass_sum adds the volume and phi of the current state
+// into acc[2] (fp64); cl_mass_check compares the volume with the previous prediction and
+// sets the fallback flag when the relative error exceeds `limit`.
+extern "C" __global__ void cl_mass_sum(fip_grid g, int par, double* acc) {
+  if (g.ctl->halt_req[par ^ 1]) return;
+  const int cur = par ^ 1;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  double v = 0.0, f = 0.0;
+  if (x >= 1 && x <= g.W - 2 && y >= 1 && y <= g.H - 2) v = (double)g.h[cur][(long)y * g.P + x];
+  if (x == 0 && y >= 1 && y <= g.H - 2)
+    f = (double)g.qx[cur][(long)y * g.P + g.W - 2] - (double)g.qx[cur][(long)y * g.P];
+  if (y == 0 && x >= 1 && x <= g.W - 2)
+    f += (double)g.qy[cur][(long)(g.H - 2) * g.P + x] - (double)g.qy[cur][x];
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = v;
+  sh[1][threadIdx.x] = f;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&acc[0], sh[0][0]);
+    atomicAdd(&acc[1], sh[1][0]);
+  }
+}
+
+extern "C" __global__ void cl_mass_check(fip_grid g, int par, double* acc, ClMass* ms, double limit) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (g.ctl->halt_req[par ^ 1]) return;
+  const double dx = g.dx;
+  const double vol = acc[0] * dx * dx;
+  const double phi = -(double)g.ctl->dt * dx * acc[1];
+  if (ms->valid) {
+    const double pred = ms->v_prev + ms->phi;
+    const double rel = fabs(vol - pred) / (vol > 1e-30 ? vol : 1e-30);
+    ms->last_rel = rel;
+    if (rel > ms->max_rel) ms->max_rel = rel;
+    if (rel > limit && vol > 0.0) ms->fallback = 1;
+  }
+  ms->v_prev = vol;
+  ms->phi = phi;
+  ms->valid = 1;
+  ms->steps = ms->steps + 1;
+  acc[0] = 0.0;
+  acc[1] = 0.0;
+}
+
+// Ensemble forms of cl_correct / cl_mass_sum / cl_mass_check: blockIdx.z (or the thread
+// of cl_mass_check_ens) = scenario; o holds one W x H image per scenario.
+extern "C" __global__ void cl_correct_ens(const fip_ens_scn* scn, int par, const float* o, float bound,
+                                          float sat, ClMass* ms) {
+  const int s = blockIdx.z;
+  const fip_grid& g = scn[s].g;
+  if (g.ctl->halt_req[par ^ 1] || ms[s].fallback) return;
+  const int cur = par ^ 1;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= g.W || y >= g.H) return;
+  const float* os = o + (long)s * g.W * g.H * 2;
+  const long p = (long)y * g.W + x, i = (long)y * g.P + x;
+  unsigned nsat = 0;
+  if (x >= 1 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) {
+    const float ox = os[p * 2];
+    const float q = g.qx[cur][i];
+    if (fabsf(ox) <= sat) g.qx[cur][i] = q + tanhf(ox) * bound * fabsf(q);
+    else nsat++;
+  }
+  if (x >= 2 && x <= g.W - 3 && y >= 1 && y <= g.H - 3) {
+    const float oy = os[p * 2 + 1];
+    const float q = g.qy[cur][i];
+    if (fabsf(oy) <= sat) g.qy[cur][i] = q + tanhf(oy) * bound * fabsf(q);
+    else nsat++;
+  }
+  if (nsat) atomicAdd(&ms[s].nsat, nsat);
+}
+
+extern "C" __global__ void cl_mass_sum_ens(const fip_ens_scn* scn, int par, double* acc) {
+  const int s = blockIdx.z;
+  const fip_grid& g = scn[s].g;
+  if (g.ctl->halt_req[par ^ 1]) return;
+  const int cur = par ^ 1;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  double v = 0.0, f = 0.0;
+  if (x >= 1 && x <= g.W - 2 && y >= 1 && y <= g.H - 2) v = (double)g.h[cur][(long)y * g.P + x];
+  if (x == 0 && y >= 1 && y <= g.H - 2)
+    f = (double)g.qx[cur][(long)y * g.P + g.W - 2] - (double)g.qx[cur][(long)y * g.P];
+  if (y == 0 && x >= 1 && x <= g.W - 2)
+    f += (double)g.qy[cur][(long)(g.H - 2) * g.P + x] - (double)g.qy[cur][x];
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = v;
+  sh[1][threadIdx.x] = f;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&acc[2 * s], sh[0][0]);
+    atomicAdd(&acc[2 * s + 1], sh[1][0]);
+  }
+}
+
+extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms,
+                                             double limit) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const fip_grid& g = scn[s].g;
+  if (g.ctl->halt_req[par ^ 1]) return;
+  const double dx = g.dx;
+  const double vol = acc[2 * s] * dx * dx;
+  const double phi = -(double)g.ctl->dt * dx * acc[2 * s + 1];
+  ClMass& m = ms[s];
+  if (m.valid) {
+    const double pred = m.v_prev + m.phi;
+    const double rel = fabs(vol - pred) / (vol > 1e-30 ? vol : 1e-30);
+    m.last_rel = rel;
+    if (rel > m.max_rel) m.max_rel = rel;
+    if (rel > limit && vol > 0.0) m.fallback = 1;
+  }
+  m.v_prev = vol;
+  m.phi = phi;
+  m.valid = 1;
+  m.steps = m.steps + 1;
+  acc[2 * s] = 0.0;
+  acc[2 * s + 1] = 0.0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Targets of a training sample: the fine domain's current face discharges restricted as
+// in cl_restrict, [pixel][2] fp32, and the mask of trainable faces (the ones cl_correct
+// corrects).
+extern "C" __global__ void cl_restrict_flux(fip_grid f, int fcur, int F, int Wc, int Hc, float* qt,
+                                            uint8_t* mask) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
+  if (I >= Wc || J >= Hc) return;
+  double sx = 0.0, sy = 0.0;
+  for (int r = 0; r < F; r++) sx += (double)f.qx[fcur][(long)(F * J + r) * f.P + F * I + F - 1];
+  for (int q = 0; q < F; q++) sy += (double)f.qy[fcur][(long)(F * J + F - 1) * f.P + F * I + q];
+  const long p = (long)J * Wc + I;
+  qt[p * 2] = (float)(sx / F);
+  qt[p * 2 + 1] = (float)(sy / F);
+  mask[p * 2] = (I >= 1 && I <= Wc - 3 && J >= 2 && J <= Hc - 3) ? 1 : 0;
+  mask[p * 2 + 1] = (I >= 2 && I <= Wc - 3 && J >= 1 && J <= Hc - 3) ? 1 : 0;
+}
+
+// The current face discharges of a domain, [pixel][2] fp32 (the plain step's result).
+extern "C" __global__ void cl_flux(fip_grid g, int cur, float* q) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= g.W || y >= g.H) return;
+  const long p = (long)y * g.W + x, i = (long)y * g.P + x;
+  q[p * 2] = g.qx[cur][i];
+  q[p * 2 + 1] = g.qy[cur][i];
+}
+
+// Patches of the per-pixel sample arrays (qp, qt fp32 [pixel][2], mask u8 [pixel][2]).
+extern "C" __global__ void cl_gather_targets(const float* qp, const float* qt, const uint8_t* mask,
+                                             long sample_pixels, int W, const int* origin, int ps,
+                                             float* bqp, float* bqt, uint8_t* bmask) {
+  const int b = blockIdx.z, y = blockIdx.y, x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= ps) return;
+  const int s = origin[3 * b], x0 = origin[3 * b + 1], y0 = origin[3 * b + 2];
+  const long src = s * sample_pixels + (long)(y0 + y) * W + (x0 + x);
+  const long dst = ((long)b * gridDim.y + y) * ps + x;
+  for (int k = 0; k < 2; k++) {
+    bqp[dst * 2 + k] = qp[src * 2 + k];
+    bqt[dst * 2 + k] = qt[src * 2 + k];
+    bmask[dst * 2 + k] = mask[src * 2 + k];
+  }
+}
+
+// fp32 features [pixel][6] (host-provided, for tests) into NHWC bf16 with 32 channels.
+extern "C" __global__ void cl_feat_from_float(const float* f, long npix, u16* out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  for (int k = 0; k < CL_C; k++) out[p * CL_C + k] = k < CL_FEAT ? f2bf(f[p * CL_FEAT + k]) : (u16)0;
+}
+
+// Features of every scenario of an ensemble (blockIdx.z = scenario) from its current side,
+// NHWC bf16 with 32 channels, one image per scenario (all of one size).
+extern "C" __global__ void cl_features_ens(const fip_ens_scn* scn, const float* bstd, const float* zref,
+                                           int par, u16* out) {
+  const int s = blockIdx.z;
+  const fip_grid& g = scn[s].g;
+  if (g.ctl->halt_req[par ^ 1]) return;
+  const int cur = par ^ 1;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= g.W || y >= g.H) return;
+  ClFeat f{g.h[cur], g.qx[cur], g.qy[cur], g.bed, g.kst, bstd + (long)s * g.W * g.H, g.kst_uniform,
+           zref[s], g.W, g.H, g.P};
+  float v[CL_FEAT];
+  feat_pixel(f, x, y, v);
+  u16* p = out + (((long)s * g.H + y) * g.W + x) * CL_C;
+  for (int k = 0; k < CL_C; k++) p[k] = k < CL_FEAT ? f2bf(v[k]) : (u16)0;
+}
+
+// ---------------------------------------------------------------------------------------
+// launchers (host side of this translation unit)
+hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
+                          u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,
+                          hipStream_t st) {
+  const dim3 grid((W + 63) / 64, (H + 3) / 4, N), block(256);
+  switch (epi) {
+    case EPI_GELU:
+      hipLaunchKernelGGL(cl_conv<EPI_GELU>, grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par);
+      break;
+    case EPI_OUT:
+      hipLaunchKernelGGL(cl_conv<EPI_OUT>, grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par);
+      break;
+    default:
+      hipLaunchKernelGGL(cl_conv<EPI_DGELU>, grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par);
+      break;
+  }
+  return hipGetLastError();
+}
+
+#define CL_WRAP(name, params, args, grid, block)                          \
+  hipError_t name##_launch params {                                      \
+    hipLaunchKernelGGL(name, grid, block, 0, st, args);                   \
+    return hipGetLastError();                                             \
+  }
+#define P(...) __VA_ARGS__
+
+CL_WRAP(cl_mfma_probe, (const u16* A, const u16* B, float* D, hipStream_t st), P(A, B, D), dim3(1), dim3(64))
+CL_WRAP(cl_restrict, (fip_grid f, int fcur, fip_grid c, int ccur, int F, hipStream_t st),
+        P(f, fcur, c, ccur, F), dim3((c.W + 255) / 256, c.H), dim3(256))
+CL_WRAP(cl_restrict_flux, (fip_grid f, int fcur, int F, int Wc, int Hc, float* qt, uint8_t* mask, hipStream_t st),
+        P(f, fcur, F, Wc, Hc, qt, mask), dim3((Wc + 255) / 256, Hc), dim3(256))
+CL_WRAP(cl_flux, (fip_grid g, int cur, float* q, hipStream_t st), P(g, cur, q), dim3((g.W + 255) / 256, g.H),
+        dim3(256))
+CL_WRAP(cl_features, (ClFeat s, u16* out, int cstride, const fip_ctl* ctl, int par, hipStream_t st),
+        P(s, out, cstride, ctl, par), dim3((s.W + 255) / 256, s.H), dim3(256))
+CL_WRAP(cl_features_ens, (const fip_ens_scn* scn, const float* bstd, const float* zref, int par, u16* out, int W,
+                          int H, int n, hipStream_t st),
+        P(scn, bstd, zref, par, out), dim3((W + 255) / 256, H, n), dim3(256))
+CL_WRAP(cl_gather, (const u16* samples, long sample_elems, int W, const int* origin, int ps, int ph, u16* out,
+                    int nb, hipStream_t st),
+        P(samples, sample_elems, W, origin, ps, out), dim3((ps + 127) / 128, ph, nb), dim3(128))
+CL_WRAP(cl_gather_targets, (const float* qp, const float* qt, const uint8_t* mask, long sample_pixels, int W,
+                            const int* origin, int ps, int ph, float* bqp, float* bqt, uint8_t* bmask, int nb,
+                            hipStream_t st),
+        P(qp, qt, mask, sample_pixels, W, origin, ps, bqp, bqt, bmask), dim3((ps + 127) / 128, ph, nb),
+        dim3(128))
+CL_WRAP(cl_feat_from_float, (const float* f, long npix, u16* out, hipStream_t st), P(f, npix, out),
+        dim3((unsigned)((npix + 255) / 256)), dim3(256))
+CL_WRAP(cl_wgrad, (const u16* X, const u16* dZ, float* part, int N, int H, int W, int nblocks, hipStream_t st),
+        P(X, dZ, part, N, H, W), dim3(nblocks), dim3(256))
+CL_WRAP(cl_reduce, (const float* part, int nparts, int n, float* out, hipStream_t st), P(part, nparts, n, out),
+        dim3((n + 255) / 256), dim3(256))
+CL_WRAP(cl_loss, (const float* o, const float* qp, const float* qt, const uint8_t* mask, long npix, float bound,
+                  float inv_count, u16* dz, double* part, hipStream_t st),
+        P(o, qp, qt, mask, npix, bound, inv_count, dz, part), dim3((unsigned)((npix + 255) / 256)), dim3(256))
+CL_WRAP(cl_pack, (const float* w, int nlayers, u16* wf, u16* wb, hipStream_t st), P(w, nlayers, wf, wb),
+        dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))
+CL_WRAP(cl_adam, (float* w, float* m, float* v, const float* g, int nlayers, float lr, float b1, float b2,
+                  float eps, float c1, float c2, hipStream_t st),
+        P(w, m, v, g, nlayers, lr, b1, b2, eps, c1, c2), dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))
+CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms, hipStream_t st),
+        P(g, par, o, bound, sat, ms), dim3((g.W + 255) / 256, g.H), dim3(256))
+CL_WRAP(cl_mass_sum, (fip_grid g, int par, double* acc, hipStream_t st), P(g, par, acc),
+        dim3((g.W + 255) / 256, g.H), dim3(256))
+CL_WRAP(cl_mass_check, (fip_grid g, int par, double* acc, ClMass* ms, double limit, hipStream_t st),
+        P(g, par, acc, ms, limit), dim3(1), dim3(64))
+CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, int par, const float* o, float bound, float sat, ClMass* ms, int W,
+                         int H, int n, hipStream_t st),
+        P(scn, par, o, bound, sat, ms), dim3((W + 255) / 256, H, n), dim3(256))
+CL_WRAP(cl_mass_sum_ens, (const fip_ens_scn* scn, int par, double* acc, int W, int H, int n, hipStream_t st),
+        P(scn, par, acc), dim3((W + 255) / 256, H, n), dim3(256))
+CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
+                            hipStream_t st),
+        P(scn, n, par, acc, ms, limit), dim3((n + 63) / 64), dim3(64))

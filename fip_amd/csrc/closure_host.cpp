@@ -1,0 +1,614 @@
+// libfiphip: host side of the learned-closure C ABI (include/fip_closure.h): parameters
+// and Adam state, the training-sample store, the training and evaluation loops, and the
+// inference hook that corrects a coarse simulation (or ensemble) after every step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fip_closure.h"
+#include "closure.h"
+#include "fip_internal.h"
+
+using fip_internal::fail;
+
+#define CHIP(expr)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(FIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+namespace {
+
+constexpr float kBound = 0.1f;   // |c| <= 0.1 |q|
+constexpr int kWgradBlocks = 256;
+
+// splitmix64 stream with uniform and normal draws (Box-Muller)
+struct Rng {
+  uint64_t s;
+  uint64_t next() {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+  double normal() {
+    double u1 = uniform(), u2 = uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+};
+
+template <class T>
+int dalloc(T** p, size_t n) {
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  if (n == 0) return FIP_OK;
+  CHIP(hipMalloc(p, n * sizeof(T)));
+  return FIP_OK;
+}
+
+// Activations of one batch shape (N images of H x W), NHWC bf16 with 32 channels.
+struct Work {
+  int N = 0, H = 0, W = 0;
+  u16* x0 = nullptr;
+  u16* z[3] = {};
+  u16* a[3] = {};
+  u16* dz[4] = {};
+  float* o = nullptr;
+  float* qp = nullptr;
+  float* qt = nullptr;
+  uint8_t* mask = nullptr;
+  int* origin = nullptr;
+  double* lpart = nullptr;
+  ~Work() { release(); }
+  void release() {
+    auto f = [](void* p) {
+      if (p) hipFree(p);
+    };
+    f(x0);
+    for (auto* p : z) f(p);
+    for (auto* p : a) f(p);
+    for (auto* p : dz) f(p);
+    f(o); f(qp); f(qt); f(mask); f(origin); f(lpart);
+    x0 = nullptr; o = qp = qt = nullptr; mask = nullptr; origin = nullptr; lpart = nullptr;
+    for (auto& p : z) p = nullptr;
+    for (auto& p : a) p = nullptr;
+    for (auto& p : dz) p = nullptr;
+    N = H = W = 0;
+  }
+  long pixels() const { return (long)N * H * W; }
+  // forward tensors always; backward and target tensors when `train`
+  int ensure(int n, int h, int w, bool train) {
+    if (n == N && h == H && w == W && (!train || dz[0])) return FIP_OK;
+    release();
+    N = n; H = h; W = w;
+    const size_t act = (size_t)n * h * w * CL_C;
+    int rc = dalloc(&x0, act);
+    for (int k = 0; k < 3 && !rc; k++) rc = dalloc(&z[k], act) | dalloc(&a[k], act);
+    if (!rc) rc = dalloc(&o, (size_t)n * h * w * 2);
+    if (!rc) rc = dalloc(&lpart, (size_t)((pixels() + 255) / 256) * 3);
+    if (train) {
+      for (int k = 0; k < 4 && !rc; k++) rc = dalloc(&dz[k], act);
+      if (!rc) rc = dalloc(&qp, (size_t)n * h * w * 2);
+      if (!rc) rc = dalloc(&qt, (size_t)n * h * w * 2);
+      if (!rc) rc = dalloc(&mask, (size_t)n * h * w * 2);
+      if (!rc) rc = dalloc(&origin, (size_t)n * 3);
+    }
+    return rc;
+  }
+};
+
+struct Attachment {
+  fip_sim* sim = nullptr;   // single domain, or
+  fip_ens* ens = nullptr;   // an ensemble
+  float* bstd = nullptr;
+  float* zref = nullptr;    // device, one per scenario (ensemble)
+  float zref_h = 0.f;
+  ClMass* ms = nullptr;     // one per scenario
+  double* acc = nullptr;    // 2 per scenario
+  float sat = 2.65f;
+  double limit = 1e-4;
+  Work work;
+  ~Attachment() {
+    if (bstd) hipFree(bstd);
+    if (zref) hipFree(zref);
+    if (ms) hipFree(ms);
+    if (acc) hipFree(acc);
+  }
+};
+
+}  // namespace
+
+struct fip_closure {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  float* w = nullptr;   // fp32 master parameters [layer][CL_LAYER_PARAMS]
+  float* m = nullptr;
+  float* v = nullptr;
+  float* g = nullptr;
+  u16* wf = nullptr;    // packed forward weights [layer][CL_PACKED]
+  u16* wb = nullptr;    // packed backward-data weights
+  float* part = nullptr;
+  int t = 0;            // Adam steps taken
+  // training samples
+  int Wc = 0, Hc = 0, cap = 0, ns = 0;
+  u16* s_feat = nullptr;     // [cap][Hc][Wc][8]
+  float* s_qp = nullptr;     // [cap][Hc][Wc][2]
+  float* s_qt = nullptr;
+  uint8_t* s_mask = nullptr;
+  Work train;
+  Work eval;
+  Work host;
+  std::vector<std::unique_ptr<Attachment>> att;
+};
+
+namespace {
+
+const float* bias_of(const fip_closure* cl, int l) { return cl->w + (size_t)l * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C; }
+
+int pack(fip_closure* cl) {
+  CHIP(cl_pack_launch(cl->w, CL_LAYERS, cl->wf, cl->wb, cl->stream));
+  return FIP_OK;
+}
+
+// forward pass of ws (x0 already filled); ctl/par: skip when that domain's step was skipped
+int forward(fip_closure* cl, Work& ws, const fip_ctl* ctl, int par, hipStream_t st) {
+  const u16* in = ws.x0;
+  for (int l = 0; l < 3; l++) {
+    CHIP(cl_launch_conv(EPI_GELU, in, cl->wf + (size_t)l * CL_PACKED, bias_of(cl, l), nullptr, ws.z[l], ws.a[l],
+                        nullptr, ws.N, ws.H, ws.W, ctl, par, st));
+    in = ws.a[l];
+  }
+  CHIP(cl_launch_conv(EPI_OUT, in, cl->wf + 3 * (size_t)CL_PACKED, bias_of(cl, 3), nullptr, nullptr, nullptr, ws.o,
+                      ws.N, ws.H, ws.W, ctl, par, st));
+  return FIP_OK;
+}
+
+// loss + backward into cl->g; returns the partial sums in ws.lpart
+int backward(fip_closure* cl, Work& ws, float bound, hipStream_t st) {
+  const long npix = ws.pixels();
+  CHIP(cl_loss_launch(ws.o, ws.qp, ws.qt, ws.mask, npix, bound, 1.0f / (2.0f * (float)npix), ws.dz[3],
+                      ws.lpart, st));
+  const int nb = (int)std::min<long>(kWgradBlocks, std::max<long>(1, (long)ws.N * ((ws.H + 3) / 4) * ((ws.W + 63) / 64)));
+  const u16* xin[4] = {ws.x0, ws.a[0], ws.a[1], ws.a[2]};
+  for (int l = 3; l >= 0; l--) {
+    CHIP(cl_wgrad_launch(xin[l], ws.dz[l], cl->part, ws.N, ws.H, ws.W, nb, st));
+    CHIP(cl_reduce_launch(cl->part, nb, CL_LAYER_PARAMS, cl->g + (size_t)l * CL_LAYER_PARAMS, st));
+    if (l > 0)
+      CHIP(cl_launch_conv(EPI_DGELU, ws.dz[l], cl->wb + (size_t)l * CL_PACKED, nullptr, ws.z[l - 1], ws.dz[l - 1],
+                          nullptr, nullptr, ws.N, ws.H, ws.W, nullptr, 0, st));
+  }
+  return FIP_OK;
+}
+
+int read_loss(Work& ws, hipStream_t st, double* sum_r, double* sum_m, double* sum_p) {
+  const int nb = (int)((ws.pixels() + 255) / 256);
+  std::vector<double> h((size_t)nb * 3);
+  CHIP(hipMemcpyAsync(h.data(), ws.lpart, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  CHIP(hipStreamSynchronize(st));
+  double r = 0, m = 0, p = 0;
+  for (int b = 0; b < nb; b++) {
+    r += h[(size_t)b * 3];
+    m += h[(size_t)b * 3 + 1];
+    p += h[(size_t)b * 3 + 2];
+  }
+  *sum_r = r;
+  *sum_m = m;
+  *sum_p = p;
+  return FIP_OK;
+}
+
+int init_params(fip_closure* cl, uint64_t seed) {
+  std::vector<float> w((size_t)CL_LAYERS * CL_LAYER_PARAMS, 0.f);
+  Rng rng{seed};
+  for (int l = 0; l < CL_LAYERS; l++) {
+    const int cin = l == 0 ? CL_FEAT : CL_C, cout = l == CL_LAYERS - 1 ? CL_OUT : CL_C;
+    const double sd = std::sqrt(2.0 / (9.0 * cin)) * (l == CL_LAYERS - 1 ? 0.1 : 1.0);
+    for (int t = 0; t < CL_TAPS; t++)
+      for (int ci = 0; ci < cin; ci++)
+        for (int co = 0; co < cout; co++)
+          w[(size_t)l * CL_LAYER_PARAMS + ((size_t)t * CL_C + ci) * CL_C + co] = (float)(sd * rng.normal());
+  }
+  CHIP(hipMemcpy(cl->w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+  return pack(cl);
+}
+
+Attachment* find_att(fip_closure* cl, fip_sim* sim, fip_ens* ens) {
+  for (auto& a : cl->att)
+    if ((sim && a->sim == sim) || (ens && a->ens == ens)) return a.get();
+  return nullptr;
+}
+
+// all closures that hold an attachment of a simulation (to detach it)
+std::vector<fip_closure*>& registry() {
+  static std::vector<fip_closure*> r;
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fip_restrict(fip_sim* fine, fip_sim* coarse, int F) {
+  if (!fine || !coarse) return fail(FIP_E_INVALID, "null argument");
+  const fip_grid& f = fip_internal::grid(fine);
+  const fip_grid& c = fip_internal::grid(coarse);
+  if (F < 1 || c.W * F != f.W || c.H * F != f.H)
+    return fail(FIP_E_INVALID, "coarse grid must be exactly W/factor x H/factor of the fine grid");
+  CHIP(hipSetDevice(fip_internal::device(fine)));
+  CHIP(hipDeviceSynchronize());
+  CHIP(cl_restrict_launch(f, fip_internal::cur_side(fine), c, fip_internal::cur_side(coarse), F, nullptr));
+  CHIP(hipDeviceSynchronize());
+  return FIP_OK;
+}
+
+int fip_closure_create(int device, unsigned long long seed, int Wc, int Hc, int max_samples, fip_closure** out) {
+  if (!out) return fail(FIP_E_INVALID, "null argument");
+  *out = nullptr;
+  if (Wc < 8 || Hc < 8 || max_samples < 0) return fail(FIP_E_INVALID, "coarse grid too small");
+  fip_closure* cl = new fip_closure();
+  auto cleanup = [&](int rc) {
+    fip_closure_destroy(cl);
+    return rc;
+  };
+  cl->dev = device;
+  cl->Wc = Wc;
+  cl->Hc = Hc;
+  cl->cap = max_samples;
+  if (hipSetDevice(device) != hipSuccess) return cleanup(fail(FIP_E_HIP, "hipSetDevice failed (no GPU?)"));
+  if (hipStreamCreateWithFlags(&cl->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(FIP_E_HIP, "hipStreamCreate failed"));
+  const size_t np = (size_t)CL_LAYERS * CL_LAYER_PARAMS;
+  int rc = dalloc(&cl->w, np);
+  if (!rc) rc = dalloc(&cl->m, np);
+  if (!rc) rc = dalloc(&cl->v, np);
+  if (!rc) rc = dalloc(&cl->g, np);
+  if (!rc) rc = dalloc(&cl->wf, (size_t)CL_LAYERS * CL_PACKED);
+  if (!rc) rc = dalloc(&cl->wb, (size_t)CL_LAYERS * CL_PACKED);
+  if (!rc) rc = dalloc(&cl->part, (size_t)kWgradBlocks * CL_LAYER_PARAMS);
+  const size_t spx = (size_t)Wc * Hc;
+  if (!rc) rc = dalloc(&cl->s_feat, (size_t)max_samples * spx * 8);
+  if (!rc) rc = dalloc(&cl->s_qp, (size_t)max_samples * spx * 2);
+  if (!rc) rc = dalloc(&cl->s_qt, (size_t)max_samples * spx * 2);
+  if (!rc) rc = dalloc(&cl->s_mask, (size_t)max_samples * spx * 2);
+  if (rc) return cleanup(rc);
+  if (hipMemset(cl->m, 0, np * sizeof(float)) != hipSuccess || hipMemset(cl->v, 0, np * sizeof(float)) != hipSuccess ||
+      hipMemset(cl->wf, 0, (size_t)CL_LAYERS * CL_PACKED * 2) != hipSuccess ||
+      hipMemset(cl->wb, 0, (size_t)CL_LAYERS * CL_PACKED * 2) != hipSuccess)
+    return cleanup(fail(FIP_E_HIP, "hipMemset failed"));
+  if ((rc = init_params(cl, seed))) return cleanup(rc);
+  if (hipStreamSynchronize(cl->stream) != hipSuccess) return cleanup(fail(FIP_E_HIP, "sync failed"));
+  registry().push_back(cl);
+  *out = cl;
+  return FIP_OK;
+}
+
+int fip_closure_destroy(fip_closure* cl) {
+  if (!cl) return FIP_OK;
+  hipSetDevice(cl->dev);
+  for (auto& a : cl->att) {
+    if (a->sim) fip_internal::set_post_hook(a->sim, nullptr);
+    if (a->ens) fip_internal::ens_set_post_hook(a->ens, nullptr);
+  }
+  if (cl->stream) hipStreamSynchronize(cl->stream);
+  cl->att.clear();
+  for (void* p : {(void*)cl->w, (void*)cl->m, (void*)cl->v, (void*)cl->g, (void*)cl->wf, (void*)cl->wb,
+                  (void*)cl->part, (void*)cl->s_feat, (void*)cl->s_qp, (void*)cl->s_qt, (void*)cl->s_mask})
+    if (p) hipFree(p);
+  cl->train.release();
+  cl->eval.release();
+  cl->host.release();
+  if (cl->stream) hipStreamDestroy(cl->stream);
+  auto& r = registry();
+  r.erase(std::remove(r.begin(), r.end(), cl), r.end());
+  delete cl;
+  return FIP_OK;
+}
+
+int fip_closure_get_params(fip_closure* cl, float* params) {
+  if (!cl || !params) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(cl->dev));
+  CHIP(hipStreamSynchronize(cl->stream));
+  CHIP(hipMemcpy(params, cl->w, sizeof(float) * CL_LAYERS * CL_LAYER_PARAMS, hipMemcpyDeviceToHost));
+  return FIP_OK;
+}
+
+int fip_closure_set_params(fip_closure* cl, const float* params) {
+  if (!cl || !params) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(cl->dev));
+  CHIP(hipStreamSynchronize(cl->stream));
+  CHIP(hipMemcpy(cl->w, params, sizeof(float) * CL_LAYERS * CL_LAYER_PARAMS, hipMemcpyHostToDevice));
+  int rc = pack(cl);
+  if (rc) return rc;
+  CHIP(hipStreamSynchronize(cl->stream));
+  return FIP_OK;
+}
+
+int fip_closure_add_sample(fip_closure* cl, fip_sim* coarse, fip_sim* fine, int F, const float* bed_std,
+                           float zref) {
+  if (!cl || !coarse || !fine || !bed_std) return fail(FIP_E_INVALID, "null argument");
+  if (cl->ns >= cl->cap) return fail(FIP_E_INVALID, "sample store is full");
+  const fip_grid& c = fip_internal::grid(coarse);
+  const fip_grid& f = fip_internal::grid(fine);
+  if (c.W != cl->Wc || c.H != cl->Hc) return fail(FIP_E_INVALID, "coarse grid size differs from the closure's");
+  if (c.W * F != f.W || c.H * F != f.H) return fail(FIP_E_INVALID, "fine grid must be factor x the coarse grid");
+  CHIP(hipSetDevice(cl->dev));
+  CHIP(hipDeviceSynchronize());
+  const int cur = fip_internal::cur_side(coarse), oth = cur ^ 1;
+  const size_t spx = (size_t)c.W * c.H;
+  float* bstd = nullptr;
+  CHIP(hipMalloc(&bstd, spx * sizeof(float)));
+  CHIP(hipMemcpy(bstd, bed_std, spx * sizeof(float), hipMemcpyHostToDevice));
+  ClFeat fe{c.h[oth], c.qx[oth], c.qy[oth], c.bed, c.kst, bstd, c.kst_uniform, zref, c.W, c.H, c.P};
+  const size_t s = (size_t)cl->ns;
+  hipError_t e = cl_features_launch(fe, cl->s_feat + s * spx * 8, 8, nullptr, 0, cl->stream);
+  if (e == hipSuccess) e = cl_flux_launch(c, cur, cl->s_qp + s * spx * 2, cl->stream);
+  if (e == hipSuccess)
+    e = cl_restrict_flux_launch(f, fip_internal::cur_side(fine), F, c.W, c.H, cl->s_qt + s * spx * 2,
+                                cl->s_mask + s * spx * 2, cl->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(cl->stream);
+  hipFree(bstd);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("add_sample: ") + hipGetErrorString(e));
+  cl->ns++;
+  return FIP_OK;
+}
+
+int fip_closure_num_samples(fip_closure* cl, int* n) {
+  if (!cl || !n) return fail(FIP_E_INVALID, "null argument");
+  *n = cl->ns;
+  return FIP_OK;
+}
+
+int fip_closure_train(fip_closure* cl, int steps, int batch, int patch, float lr, int first, int n,
+                      unsigned long long seed, float* losses) {
+  if (!cl) return fail(FIP_E_INVALID, "null argument");
+  if (n < 1 || first < 0 || first + n > cl->ns) return fail(FIP_E_INVALID, "sample range outside the store");
+  if (patch < 8 || patch > std::min(cl->Wc, cl->Hc) || batch < 1)
+    return fail(FIP_E_INVALID, "patch must fit the coarse grid");
+  CHIP(hipSetDevice(cl->dev));
+  int rc = cl->train.ensure(batch, patch, patch, true);
+  if (rc) return rc;
+  Rng rng{seed};
+  std::vector<int> org((size_t)batch * 3);
+  const size_t spx = (size_t)cl->Wc * cl->Hc;
+  for (int it = 0; it < steps; it++) {
+    for (int b = 0; b < batch; b++) {
+      org[(size_t)b * 3] = first + (int)(rng.uniform() * n) % n;
+      org[(size_t)b * 3 + 1] = (int)(rng.uniform() * (cl->Wc - patch + 1)) % (cl->Wc - patch + 1);
+      org[(size_t)b * 3 + 2] = (int)(rng.uniform() * (cl->Hc - patch + 1)) % (cl->Hc - patch + 1);
+    }
+    Work& ws = cl->train;
+    CHIP(hipMemcpyAsync(ws.origin, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, cl->stream));
+    CHIP(cl_gather_launch(cl->s_feat, (long)spx * 8, cl->Wc, ws.origin, patch, patch, ws.x0, batch, cl->stream));
+    CHIP(cl_gather_targets_launch(cl->s_qp, cl->s_qt, cl->s_mask, (long)spx, cl->Wc, ws.origin, patch, patch,
+                                  ws.qp, ws.qt, ws.mask, batch, cl->stream));
+    if ((rc = forward(cl, ws, nullptr, 0, cl->stream))) return rc;
+    if ((rc = backward(cl, ws, kBound, cl->stream))) return rc;
+    cl->t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)cl->t), c2 = 1.f - std::pow(b2, (float)cl->t);
+    CHIP(cl_adam_launch(cl->w, cl->m, cl->v, cl->g, CL_LAYERS, lr, b1, b2, 1e-8f, c1, c2, cl->stream));
+    if ((rc = pack(cl))) return rc;
+    double r, m, p;
+    if ((rc = read_loss(ws, cl->stream, &r, &m, &p))) return rc;
+    if (losses) losses[it] = m > 0 ? (float)(r / m) : 0.f;
+  }
+  return FIP_OK;
+}
+
+int fip_closure_eval(fip_closure* cl, int first, int n, double* mse_plain, double* mse_learned) {
+  if (!cl) return fail(FIP_E_INVALID, "null argument");
+  if (n < 1 || first < 0 || first + n > cl->ns) return fail(FIP_E_INVALID, "sample range outside the store");
+  CHIP(hipSetDevice(cl->dev));
+  Work& ws = cl->eval;
+  int rc = ws.ensure(1, cl->Hc, cl->Wc, true);
+  if (rc) return rc;
+  const size_t spx = (size_t)cl->Wc * cl->Hc;
+  double R = 0, M = 0, Pp = 0;
+  for (int s = first; s < first + n; s++) {
+    const int org[3] = {s, 0, 0};
+    CHIP(hipMemcpyAsync(ws.origin, org, sizeof org, hipMemcpyHostToDevice, cl->stream));
+    CHIP(cl_gather_launch(cl->s_feat, (long)spx * 8, cl->Wc, ws.origin, cl->Wc, cl->Hc, ws.x0, 1, cl->stream));
+    CHIP(cl_gather_targets_launch(cl->s_qp, cl->s_qt, cl->s_mask, (long)spx, cl->Wc, ws.origin, cl->Wc, cl->Hc,
+                                  ws.qp, ws.qt, ws.mask, 1, cl->stream));
+    if ((rc = forward(cl, ws, nullptr, 0, cl->stream))) return rc;
+    CHIP(cl_loss_launch(ws.o, ws.qp, ws.qt, ws.mask, ws.pixels(), kBound, 1.f, ws.dz[3], ws.lpart, cl->stream));
+    double r, m, p;
+    if ((rc = read_loss(ws, cl->stream, &r, &m, &p))) return rc;
+    R += r;
+    M += m;
+    Pp += p;
+  }
+  if (mse_plain) *mse_plain = M > 0 ? Pp / M : 0.0;
+  if (mse_learned) *mse_learned = M > 0 ? R / M : 0.0;
+  return FIP_OK;
+}
+
+int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, float zref, float sat,
+                       double mass_limit) {
+  if (!cl || !coarse || !bed_std) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(cl->dev));
+  fip_closure_detach(coarse);
+  auto a = std::make_unique<Attachment>();
+  a->sim = coarse;
+  a->zref_h = zref;
+  a->sat = sat;
+  a->limit = mass_limit;
+  const fip_grid& g = fip_internal::grid(coarse);
+  const size_t spx = (size_t)g.W * g.H;
+  int rc = dalloc(&a->bstd, spx);
+  if (!rc) rc = dalloc(&a->ms, 1);
+  if (!rc) rc = dalloc(&a->acc, 2);
+  if (!rc) rc = a->work.ensure(1, g.H, g.W, false);
+  if (rc) return rc;
+  CHIP(hipMemcpy(a->bstd, bed_std, spx * sizeof(float), hipMemcpyHostToDevice));
+  CHIP(hipMemset(a->ms, 0, sizeof(ClMass)));
+  CHIP(hipMemset(a->acc, 0, 2 * sizeof(double)));
+  Attachment* ap = a.get();
+  cl->att.push_back(std::move(a));
+  fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
+    fip_grid& g = fip_internal::grid_mut(ap->sim);
+    const int cur = par ^ 1;
+    ClFeat fe{g.h[cur], g.qx[cur], g.qy[cur], g.bed, g.kst, ap->bstd, g.kst_uniform, ap->zref_h, g.W, g.H, g.P};
+    CHIP(cl_features_launch(fe, ap->work.x0, CL_C, g.ctl, par, st));
+    int rc = forward(cl, ap->work, g.ctl, par, st);
+    if (rc) return rc;
+    CHIP(cl_correct_launch(g, par, ap->work.o, kBound, ap->sat, ap->ms, st));
+    CHIP(cl_mass_sum_launch(g, par, ap->acc, st));
+    CHIP(cl_mass_check_launch(g, par, ap->acc, ap->ms, ap->limit, st));
+    return FIP_OK;
+  });
+  return FIP_OK;
+}
+
+int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_std, const float* zref, float sat,
+                                double mass_limit) {
+  if (!cl || !ens || !bed_std || !zref) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(cl->dev));
+  const int n = fip_internal::ens_count(ens);
+  const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ens, 0));
+  for (int s = 1; s < n; s++) {
+    const fip_grid& g = fip_internal::grid(fip_internal::ens_member(ens, s));
+    if (g.W != g0.W || g.H != g0.H) return fail(FIP_E_INVALID, "every scenario must have one coarse size");
+  }
+  for (auto it = cl->att.begin(); it != cl->att.end(); ++it)
+    if ((*it)->ens == ens) {
+      cl->att.erase(it);
+      break;
+    }
+  auto a = std::make_unique<Attachment>();
+  a->ens = ens;
+  a->sat = sat;
+  a->limit = mass_limit;
+  const size_t spx = (size_t)g0.W * g0.H;
+  int rc = dalloc(&a->bstd, spx * n);
+  if (!rc) rc = dalloc(&a->zref, (size_t)n);
+  if (!rc) rc = dalloc(&a->ms, (size_t)n);
+  if (!rc) rc = dalloc(&a->acc, (size_t)2 * n);
+  if (!rc) rc = a->work.ensure(n, g0.H, g0.W, false);
+  if (rc) return rc;
+  CHIP(hipMemcpy(a->bstd, bed_std, spx * n * sizeof(float), hipMemcpyHostToDevice));
+  CHIP(hipMemcpy(a->zref, zref, n * sizeof(float), hipMemcpyHostToDevice));
+  CHIP(hipMemset(a->ms, 0, n * sizeof(ClMass)));
+  CHIP(hipMemset(a->acc, 0, 2 * n * sizeof(double)));
+  Attachment* ap = a.get();
+  cl->att.push_back(std::move(a));
+  fip_internal::ens_set_post_hook(ens, [cl, ap, n](hipStream_t st, int par) -> int {
+    const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ap->ens, 0));
+    CHIP(cl_features_ens_launch(fip_internal::ens_scn(ap->ens), ap->bstd, ap->zref, par, ap->work.x0, g0.W, g0.H,
+                                n, st));
+    int rc = forward(cl, ap->work, nullptr, 0, st);
+    if (rc) return rc;
+    const fip_ens_scn* scn = fip_internal::ens_scn(ap->ens);
+    CHIP(cl_correct_ens_launch(scn, par, ap->work.o, kBound, ap->sat, ap->ms, g0.W, g0.H, n, st));
+    CHIP(cl_mass_sum_ens_launch(scn, par, ap->acc, g0.W, g0.H, n, st));
+    CHIP(cl_mass_check_ens_launch(scn, n, par, ap->acc, ap->ms, ap->limit, st));
+    return FIP_OK;
+  });
+  return FIP_OK;
+}
+
+int fip_closure_detach(fip_sim* coarse) {
+  if (!coarse) return fail(FIP_E_INVALID, "null argument");
+  fip_internal::set_post_hook(coarse, nullptr);
+  for (fip_closure* cl : registry())
+    for (auto it = cl->att.begin(); it != cl->att.end(); ++it)
+      if ((*it)->sim == coarse) {
+        cl->att.erase(it);
+        break;
+      }
+  return FIP_OK;
+}
+
+int fip_closure_status(fip_closure* cl, fip_sim* coarse, fip_ens* ens, int s, int* fallback, unsigned* nsat,
+                       double* max_rel) {
+  if (!cl) return fail(FIP_E_INVALID, "null argument");
+  Attachment* a = find_att(cl, coarse, ens);
+  if (!a) return fail(FIP_E_STATE, "not attached");
+  if (ens && (s < 0 || s >= fip_internal::ens_count(ens))) return fail(FIP_E_INVALID, "scenario out of range");
+  CHIP(hipSetDevice(cl->dev));
+  CHIP(hipDeviceSynchronize());
+  ClMass m;
+  CHIP(hipMemcpy(&m, a->ms + (ens ? s : 0), sizeof m, hipMemcpyDeviceToHost));
+  if (fallback) *fallback = m.fallback;
+  if (nsat) *nsat = m.nsat;
+  if (max_rel) *max_rel = m.max_rel;
+  return FIP_OK;
+}
+
+int fip_closure_forward(fip_closure* cl, const float* feat, int N, int H, int W, float* out) {
+  if (!cl || !feat || !out) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(cl->dev));
+  Work& ws = cl->host;
+  int rc = ws.ensure(N, H, W, true);
+  if (rc) return rc;
+  const long npix = ws.pixels();
+  float* df = nullptr;
+  CHIP(hipMalloc(&df, npix * CL_FEAT * sizeof(float)));
+  CHIP(hipMemcpy(df, feat, npix * CL_FEAT * sizeof(float), hipMemcpyHostToDevice));
+  hipError_t e = cl_feat_from_float_launch(df, npix, ws.x0, cl->stream);
+  if (e == hipSuccess) rc = forward(cl, ws, nullptr, 0, cl->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(cl->stream);
+  hipFree(df);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("forward: ") + hipGetErrorString(e));
+  if (rc) return rc;
+  CHIP(hipMemcpy(out, ws.o, npix * 2 * sizeof(float), hipMemcpyDeviceToHost));
+  return FIP_OK;
+}
+
+int fip_closure_backward(fip_closure* cl, const float* feat, int N, int H, int W, const float* qp, const float* qt,
+                         const unsigned char* mask, float bound, float* grad, double* loss) {
+  if (!cl || !feat || !qp || !qt || !mask || !grad) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(cl->dev));
+  Work& ws = cl->host;
+  int rc = ws.ensure(N, H, W, true);
+  if (rc) return rc;
+  const long npix = ws.pixels();
+  float* df = nullptr;
+  CHIP(hipMalloc(&df, npix * CL_FEAT * sizeof(float)));
+  CHIP(hipMemcpy(df, feat, npix * CL_FEAT * sizeof(float), hipMemcpyHostToDevice));
+  CHIP(hipMemcpy(ws.qp, qp, npix * 2 * sizeof(float), hipMemcpyHostToDevice));
+  CHIP(hipMemcpy(ws.qt, qt, npix * 2 * sizeof(float), hipMemcpyHostToDevice));
+  CHIP(hipMemcpy(ws.mask, mask, npix * 2, hipMemcpyHostToDevice));
+  hipError_t e = cl_feat_from_float_launch(df, npix, ws.x0, cl->stream);
+  if (e == hipSuccess) rc = forward(cl, ws, nullptr, 0, cl->stream);
+  if (e == hipSuccess && !rc) rc = backward(cl, ws, bound, cl->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(cl->stream);
+  hipFree(df);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("backward: ") + hipGetErrorString(e));
+  if (rc) return rc;
+  double r, m, p;
+  if ((rc = read_loss(ws, cl->stream, &r, &m, &p))) return rc;
+  if (loss) *loss = r / (2.0 * (double)npix);
+  CHIP(hipMemcpy(grad, cl->g, sizeof(float) * CL_LAYERS * CL_LAYER_PARAMS, hipMemcpyDeviceToHost));
+  return FIP_OK;
+}
+
+int fip_mfma_probe(int device, const unsigned short* A, const unsigned short* B, float* D) {
+  if (!A || !B || !D) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(device));
+  u16 *dA = nullptr, *dB = nullptr;
+  float* dD = nullptr;
+  CHIP(hipMalloc(&dA, 16 * 32 * 2));
+  CHIP(hipMalloc(&dB, 32 * 16 * 2));
+  CHIP(hipMalloc(&dD, 16 * 16 * 4));
+  CHIP(hipMemcpy(dA, A, 16 * 32 * 2, hipMemcpyHostToDevice));
+  CHIP(hipMemcpy(dB, B, 32 * 16 * 2, hipMemcpyHostToDevice));
+  CHIP(cl_mfma_probe_launch(dA, dB, dD, nullptr));
+  CHIP(hipDeviceSynchronize());
+  CHIP(hipMemcpy(D, dD, 16 * 16 * 4, hipMemcpyDeviceToHost));
+  hipFree(dA);
+  hipFree(dB);
+  hipFree(dD);
+  return FIP_OK;
+}
+
+}  // extern "C"

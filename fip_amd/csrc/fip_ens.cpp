@@ -79,6 +79,7 @@ struct fip_ens {
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   fip_stats stats{};
+  fip_internal::StepHook post_hook;  // extra work after each ensemble step
 };
 
 namespace {
@@ -227,6 +228,7 @@ int launch_flood(fip_ens* e) {
 int one_step(fip_ens* e, int par) {
   int rc = launch_stencil(e, par);
   if (!rc) rc = launch_post_apply(e, 1, par, par ^ 1);
+  if (!rc && e->post_hook) rc = e->post_hook(e->stream, par);
   return rc;
 }
 
@@ -252,6 +254,17 @@ int check_scn(fip_ens* e, int s) {
 }
 
 }  // namespace
+
+namespace fip_internal {
+void ens_set_post_hook(fip_ens* e, StepHook hook) {
+  e->post_hook = std::move(hook);
+  destroy_graphs(e);
+}
+const fip_ens_scn* ens_scn(const fip_ens* e) { return e->d_scn; }
+int ens_count(const fip_ens* e) { return e->n; }
+fip_sim* ens_member(const fip_ens* e, int s) { return e->sims[s]; }
+hipStream_t ens_stream(const fip_ens* e) { return e->stream; }
+}  // namespace fip_internal
 
 extern "C" {
 

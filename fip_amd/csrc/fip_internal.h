@@ -3,10 +3,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
 #include "../../include/fip.h"
+#include "../../include/fip_ensemble.h"
 #include "fip_ctl.h"
 
 namespace fip_internal {
@@ -24,5 +26,21 @@ int refresh(fip_sim* s);
 // the number of stages.
 int bc_stages(const fip_bc* bcs, int n, int W, int H, std::vector<int>& stage,
               std::vector<int>& lim);
+
+// Work launched on the simulation's stream after every step of the device-resident loop
+// (captured into its graphs): hook(stream, par) for the step of parity par. An empty
+// hook removes it.
+using StepHook = std::function<int(hipStream_t, int)>;
+void set_post_hook(fip_sim* s, StepHook hook);
+hipStream_t stream(const fip_sim* s);
+int cur_side(const fip_sim* s);  // the current ("1") side as the host knows it
+fip_grid& grid_mut(fip_sim* s);
+
+// ensembles
+void ens_set_post_hook(fip_ens* e, StepHook hook);
+const fip_ens_scn* ens_scn(const fip_ens* e);
+int ens_count(const fip_ens* e);
+fip_sim* ens_member(const fip_ens* e, int s);
+hipStream_t ens_stream(const fip_ens* e);
 
 }  // namespace fip_internal

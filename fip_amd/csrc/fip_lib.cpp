@@ -90,6 +90,7 @@ struct fip_sim {
   int npoints = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graph_failed = false;
+  fip_internal::StepHook post_hook;  // extra work after each step of the device-resident loop
   // stats / timing
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -471,7 +472,9 @@ void destroy_graphs(fip_sim* s) {
 int one_step(fip_sim* s, int par) {
   int rc = launch_stencil(s, par);
   if (rc) return rc;
-  return launch_bc(s, BC_POST, par);
+  rc = launch_bc(s, BC_POST, par);
+  if (rc) return rc;
+  return s->post_hook ? s->post_hook(s->stream, par) : FIP_OK;
 }
 
 // Captures kStepsPerGraph steps starting at parity par, plus the flood-plain kernel.
@@ -1105,6 +1108,14 @@ int fail(int code, const std::string& msg) { return ::fail(code, msg); }
 const fip_grid& grid(const fip_sim* s) { return s->g; }
 int staggered(const fip_sim* s) { return s->d.staggered; }
 int device(const fip_sim* s) { return s->dev; }
+
+void set_post_hook(fip_sim* s, StepHook hook) {
+  s->post_hook = std::move(hook);
+  destroy_graphs(s);
+}
+hipStream_t stream(const fip_sim* s) { return s->stream; }
+int cur_side(const fip_sim* s) { return s->host_iter & 1; }
+fip_grid& grid_mut(fip_sim* s) { return s->g; }
 
 int refresh(fip_sim* s) {
   int it = 0;

@@ -22,6 +22,12 @@ EXPORTS = ["fip_last_error", "fip_create", "fip_destroy", "fip_bc_generate", "fi
            "fip_set_dt", "fip_download", "fip_upload", "fip_tiles", "fip_get_masks",
            "fip_get_cfl", "fip_set_launch", "fip_tune", "fip_set_timing", "fip_get_stats",
            "fip_sync"]
+CLOSURE_EXPORTS = ["fip_restrict", "fip_closure_create", "fip_closure_destroy", "fip_closure_get_params",
+                   "fip_closure_set_params", "fip_closure_add_sample", "fip_closure_num_samples",
+                   "fip_closure_train", "fip_closure_eval", "fip_closure_attach",
+                   "fip_closure_attach_ensemble", "fip_closure_detach", "fip_closure_status",
+                   "fip_closure_forward", "fip_closure_backward", "fip_mfma_probe"]
+CL_PARAMS = 4 * (9 * 32 * 32 + 32)
 ENS_EXPORTS = ["fip_ens_create", "fip_ens_destroy", "fip_ens_set_bcs", "fip_ens_set_run",
                "fip_ens_simulate", "fip_ens_get_time", "fip_ens_download", "fip_ens_set_launch",
                "fip_ens_tune",
@@ -104,6 +110,25 @@ def lib():
         L.fip_ens_tune.argtypes = [C.POINTER(FipDesc), C.c_int, C.POINTER(C.POINTER(FipBC)),
                                    ip, C.c_int, C.POINTER(FipTune)]
         L.fip_set_launch.argtypes = [vp, C.c_int, C.c_int]
+        fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+        L.fip_restrict.argtypes = [vp, vp, C.c_int]
+        L.fip_closure_create.argtypes = [C.c_int, C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+        L.fip_closure_destroy.argtypes = [vp]
+        L.fip_closure_get_params.argtypes = [vp, fp]
+        L.fip_closure_set_params.argtypes = [vp, fp]
+        L.fip_closure_add_sample.argtypes = [vp, vp, vp, C.c_int, fp, C.c_float]
+        L.fip_closure_num_samples.argtypes = [vp, ip]
+        L.fip_closure_train.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                        C.c_ulonglong, fp]
+        L.fip_closure_eval.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+        L.fip_closure_attach.argtypes = [vp, vp, fp, C.c_float, C.c_float, C.c_double]
+        L.fip_closure_attach_ensemble.argtypes = [vp, vp, fp, fp, C.c_float, C.c_double]
+        L.fip_closure_detach.argtypes = [vp]
+        L.fip_closure_status.argtypes = [vp, vp, vp, C.c_int, ip, C.POINTER(C.c_uint), dp]
+        L.fip_closure_forward.argtypes = [vp, fp, C.c_int, C.c_int, C.c_int, fp]
+        L.fip_closure_backward.argtypes = [vp, fp, C.c_int, C.c_int, C.c_int, fp, fp,
+                                           C.POINTER(C.c_ubyte), C.c_float, fp, dp]
+        L.fip_mfma_probe.argtypes = [C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), fp]
         L.fip_tune.argtypes = [C.POINTER(FipDesc), C.POINTER(FipBC), C.c_int, C.c_int,
                                C.POINTER(FipTune)]
         L.fip_ens_set_timing.argtypes = [vp, C.c_int]
@@ -415,3 +440,116 @@ def tune_ensemble(cases, mode: str = "masked", variant: str = "fast", steps: int
     r = FipTune()
     _check(lib().fip_ens_tune(descs, len(cases), ptrs, nbcs, int(steps), C.byref(r)))
     return _tune_dict(r)
+
+
+def _f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def restrict(fine: "Sim", coarse: "Sim", factor: int = 4):
+    """fip_restrict: the fine simulation's current state onto the coarse one."""
+    _check(lib().fip_restrict(fine.s, coarse.s, int(factor)))
+
+
+def mfma_probe(A, B, device=0):
+    A = np.ascontiguousarray(A, dtype=np.uint16)
+    B = np.ascontiguousarray(B, dtype=np.uint16)
+    D = np.zeros((16, 16), np.float32)
+    _check(lib().fip_mfma_probe(device, A.ctypes.data_as(C.POINTER(C.c_ushort)),
+                                B.ctypes.data_as(C.POINTER(C.c_ushort)), _f32p(D)))
+    return D
+
+
+class Closure:
+    """The learned subgrid closure (include/fip_closure.h)."""
+
+    def __init__(self, Wc, Hc, max_samples=0, seed=1, device=0):
+        h = C.c_void_p()
+        _check(lib().fip_closure_create(device, seed, Wc, Hc, max_samples, C.byref(h)))
+        self.c = h
+        self.Wc, self.Hc = Wc, Hc
+
+    def close(self):
+        if getattr(self, "c", None):
+            lib().fip_closure_destroy(self.c)
+            self.c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = np.zeros(CL_PARAMS, np.float32)
+        _check(lib().fip_closure_get_params(self.c, _f32p(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == CL_PARAMS
+        _check(lib().fip_closure_set_params(self.c, _f32p(p)))
+
+    def add_sample(self, coarse: "Sim", fine: "Sim", bed_std, zref, factor=4):
+        b = np.ascontiguousarray(bed_std, dtype=np.float32)
+        _check(lib().fip_closure_add_sample(self.c, coarse.s, fine.s, factor, _f32p(b), float(zref)))
+
+    def num_samples(self):
+        n = C.c_int()
+        _check(lib().fip_closure_num_samples(self.c, C.byref(n)))
+        return n.value
+
+    def train(self, steps, batch=32, patch=128, lr=1e-3, first=0, n=None, seed=7):
+        n = self.num_samples() - first if n is None else n
+        out = np.zeros(steps, np.float32)
+        _check(lib().fip_closure_train(self.c, steps, batch, patch, lr, first, n, seed, _f32p(out)))
+        return out
+
+    def eval(self, first=0, n=None):
+        n = self.num_samples() - first if n is None else n
+        a, b = C.c_double(), C.c_double()
+        _check(lib().fip_closure_eval(self.c, first, n, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def attach(self, coarse: "Sim", bed_std, zref, sat=2.65, mass_limit=1e-4):
+        b = np.ascontiguousarray(bed_std, dtype=np.float32)
+        _check(lib().fip_closure_attach(self.c, coarse.s, _f32p(b), float(zref), float(sat),
+                                        float(mass_limit)))
+
+    def attach_ensemble(self, ens: "Ensemble", bed_std, zref, sat=2.65, mass_limit=1e-4):
+        b = np.ascontiguousarray(bed_std, dtype=np.float32)
+        z = np.ascontiguousarray(zref, dtype=np.float32)
+        _check(lib().fip_closure_attach_ensemble(self.c, ens.e, _f32p(b), _f32p(z), float(sat),
+                                                 float(mass_limit)))
+
+    @staticmethod
+    def detach(coarse: "Sim"):
+        _check(lib().fip_closure_detach(coarse.s))
+
+    def status(self, coarse: "Sim" = None, ens: "Ensemble" = None, s=0):
+        fb, ns, mr = C.c_int(), C.c_uint(), C.c_double()
+        _check(lib().fip_closure_status(self.c, coarse.s if coarse else None, ens.e if ens else None, s,
+                                        C.byref(fb), C.byref(ns), C.byref(mr)))
+        return {"fallback": fb.value, "saturated_faces": ns.value, "max_rel_mass_error": mr.value}
+
+    def forward(self, feat):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        N, H, W, _ = f.shape
+        out = np.zeros((N, H, W, 2), np.float32)
+        _check(lib().fip_closure_forward(self.c, _f32p(f), N, H, W, _f32p(out)))
+        return out
+
+    def backward(self, feat, qp, qt, mask, bound=0.1):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        N, H, W, _ = f.shape
+        qp = np.ascontiguousarray(qp, dtype=np.float32)
+        qt = np.ascontiguousarray(qt, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        g = np.zeros(CL_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(lib().fip_closure_backward(self.c, _f32p(f), N, H, W, _f32p(qp), _f32p(qt),
+                                          m.ctypes.data_as(C.POINTER(C.c_ubyte)), float(bound), _f32p(g),
+                                          C.byref(loss)))
+        return loss.value, g

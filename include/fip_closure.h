@@ -1,0 +1,95 @@
+/* C ABI of libfiphip: the learned subgrid closure that lets the solver run on a grid
+ * `factor` times coarser (factor 4: dx 1 m -> 4 m).
+ *
+ * A small CNN (4 layers of 3x3 convolutions, 32 channels, GELU) reads six channels of a
+ * coarse state (z, qx, qy, bed, subgrid bed std, kSt) and outputs two corrections of the
+ * coarse face discharges, c = tanh(o) * 0.1 * |q|. Corrections are added to interior
+ * faces only, so water only moves between cells of the domain: mass is conserved by
+ * construction. It is trained on the GPU (bf16 matrix-core convolutions, fp32
+ * accumulation, Adam) on pairs the solver itself produces: a restricted fine state, one
+ * plain coarse step from it, and the restricted fine state one coarse step later.
+ *
+ * The reference has no counterpart (it solves one grid at one resolution,
+ * src/kernels.cu:264-438); the restriction and the closure are pinned by this project's
+ * own numpy oracle (closed forms, invariants, finite-difference gradients).
+ * Return codes and fip_last_error() as in fip.h.
+ */
+#ifndef FIP_CLOSURE_H
+#define FIP_CLOSURE_H
+#include "fip.h"
+#include "fip_ensemble.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIP_CL_LAYERS 4
+#define FIP_CL_CH 32
+/* padded parameter layout: per layer W[tap 0..8][ci 0..31][co 0..31] then bias[co 0..31];
+ * layer 0 reads channels 0..5 only, layer 3 writes channels 0..1 only (others are 0) */
+#define FIP_CL_LAYER_PARAMS (9 * 32 * 32 + 32)
+#define FIP_CL_PARAMS (FIP_CL_LAYERS * FIP_CL_LAYER_PARAMS)
+
+typedef struct fip_closure fip_closure;
+
+/* Restriction of the fine simulation's current state onto the coarse simulation (both
+ * ping-pong sides): depth = block mean, face discharge = mean of the `factor` fine faces
+ * of the coarse face (the total discharge is kept); the coarse duration becomes the fine
+ * one. The coarse grid must be exactly W/factor x H/factor. */
+int fip_restrict(fip_sim* fine, fip_sim* coarse, int factor);
+
+/* The network (He-initialised from seed, last layer scaled by 0.1) and room for
+ * `max_samples` training samples of a Wc x Hc coarse grid. */
+int fip_closure_create(int device, unsigned long long seed, int Wc, int Hc, int max_samples,
+                       fip_closure** out);
+int fip_closure_destroy(fip_closure* cl);
+int fip_closure_get_params(fip_closure* cl, float* params);
+int fip_closure_set_params(fip_closure* cl, const float* params);
+
+/* One training sample. `coarse` has just taken one plain step from a restricted state: its
+ * other side holds that state (the input), its current side the plain result. `fine` has
+ * advanced by the same time: its restriction is the target. bed_std: subgrid std of the
+ * fine bed per coarse cell (Wc x Hc); zref: reference level of the z and bed features. */
+int fip_closure_add_sample(fip_closure* cl, fip_sim* coarse, fip_sim* fine, int factor,
+                           const float* bed_std, float zref);
+int fip_closure_num_samples(fip_closure* cl, int* n);
+
+/* `steps` Adam steps (lr, beta 0.9/0.999) on batches of `batch` random patch x patch
+ * crops from samples [first, first + n); losses[s] = the batch loss (mean squared error
+ * of the corrected coarse face discharges against the targets, m^4/s^2) of step s. */
+int fip_closure_train(fip_closure* cl, int steps, int batch, int patch, float lr, int first, int n,
+                      unsigned long long seed, float* losses);
+/* Mean squared face-discharge error over samples [first, first + n): of the plain coarse
+ * step and of the corrected one. */
+int fip_closure_eval(fip_closure* cl, int first, int n, double* mse_plain, double* mse_learned);
+
+/* Inference: after every step of fip_simulate on `coarse`, its face discharges are
+ * corrected (faces where |o| > sat keep the plain step), and the mass balance is checked;
+ * a relative error above mass_limit switches the domain to the plain step for good. */
+int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, float zref,
+                       float sat, double mass_limit);
+/* The same for every scenario of an ensemble (all scenarios one coarse size);
+ * bed_std holds n x Wc x Hc values, zref n values. */
+int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_std,
+                                const float* zref, float sat, double mass_limit);
+int fip_closure_detach(fip_sim* coarse);
+/* State of an attached domain (scenario s of an ensemble, or s = 0 with ens = NULL). */
+int fip_closure_status(fip_closure* cl, fip_sim* coarse, fip_ens* ens, int s, int* fallback,
+                       unsigned* saturated_faces, double* max_rel_mass_error);
+
+/* Test hooks on host data. fip_closure_forward: features [N][H][W][6] -> raw outputs
+ * [N][H][W][2]. fip_closure_backward: loss and gradient (FIP_CL_PARAMS) of
+ * sum m (qp + tanh(o) bound |qp| - qt)^2 / (2 N H W) for the given features, plain fluxes
+ * qp, targets qt and face mask ([N][H][W][2] each). */
+int fip_closure_forward(fip_closure* cl, const float* feat, int N, int H, int W, float* out);
+int fip_closure_backward(fip_closure* cl, const float* feat, int N, int H, int W, const float* qp,
+                         const float* qt, const unsigned char* mask, float bound, float* grad,
+                         double* loss);
+/* D = A B of one bf16 matrix-core instruction (A 16x32, B 32x16 row-major bf16 bits,
+ * D 16x16 fp32): checks the operand layout the convolutions assume. */
+int fip_mfma_probe(int device, const unsigned short* A, const unsigned short* B, float* D);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -371,7 +371,9 @@ template <class R> void stencil_warp_ref(St<R>& s) {
   const R dt_inverse = R(1.f) / dt;
   long long cells = 0, tiles = 0;
 
-#pragma omp parallel for schedule(dynamic) reduction(+ : cells, tiles) num_threads(s.threads)
+  // With the literal store predicates (quirks) two tiles can store the same cell; tiles
+  // then run in order, so the later tile's value is kept, deterministically.
+#pragma omp parallel for schedule(dynamic) reduction(+ : cells, tiles) num_threads(s.threads) if (!s.quirks)
   for (int t = 0; t < ntiles; t++) {
     const int by = t / warpsPerRow, wx = t % warpsPerRow;
     std::vector<int> col(LANES);

@@ -32,8 +32,8 @@ def test_exports_every_declared_symbol():
 
 
 def test_python_binding_covers_exports():
-    from fip_amd.sim import ENS_EXPORTS, EXPORTS
-    assert set(EXPORTS) | set(ENS_EXPORTS) == set(declared())
+    from fip_amd.sim import CLOSURE_EXPORTS, ENS_EXPORTS, EXPORTS
+    assert set(EXPORTS) | set(ENS_EXPORTS) | set(CLOSURE_EXPORTS) == set(declared())
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")

@@ -1,0 +1,130 @@
+"""Pins the closure oracle (oracle/closure_oracle.py) on the CPU: the restriction by
+closed forms and conservation, the network gradient by central finite differences in
+binary64, Adam by its first-step closed form, the metrics by hand-computed cases."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import closure_oracle as co  # noqa: E402
+
+
+def rand_params(rng, scale=0.3):
+    p = rng.normal(0.0, scale, co.NPARAMS)
+    return np.where(co.live_mask(), p, 0.0)
+
+
+def test_restrict_conserves_volume_and_discharge():
+    rng = np.random.default_rng(3)
+    F, Hc, Wc = 4, 6, 7
+    h = rng.uniform(0, 2, (Hc * F, Wc * F)).astype(np.float32)
+    qx = rng.normal(0, 1, h.shape).astype(np.float32)
+    qy = rng.normal(0, 1, h.shape).astype(np.float32)
+    hc, qxc, qyc = co.restrict(h, qx, qy, F)
+    # volume: coarse cell area F^2
+    assert abs(hc.astype(np.float64).sum() * F * F - h.astype(np.float64).sum()) < 1e-4
+    # total discharge through each coarse east face = sum over its F fine faces (width 1)
+    for J in range(Hc):
+        for I in range(Wc):
+            fine = qx[F * J:F * J + F, F * I + F - 1].astype(np.float64).sum()
+            assert abs(qxc[J, I] * F - fine) < 1e-5
+            fine = qy[F * J + F - 1, F * I:F * I + F].astype(np.float64).sum()
+            assert abs(qyc[J, I] * F - fine) < 1e-5
+
+
+def test_restrict_brute_force_and_constants():
+    F = 4
+    h = np.arange(8 * 12, dtype=np.float32).reshape(8, 12) * np.float32(0.25)
+    hc, _, _ = co.restrict(h, h, h, F)
+    for J in range(2):
+        for I in range(3):
+            blk = [float(h[F * J + r, F * I + q]) for r in range(F) for q in range(F)]
+            assert hc[J, I] == np.float32(sum(blk) / 16.0)
+    c = np.full((8, 8), 1.25, np.float32)
+    a, b, d = co.restrict(c, c, c, F)
+    assert (a == 1.25).all() and (b == 1.25).all() and (d == 1.25).all()
+    bed = np.zeros((8, 8), np.float32)
+    bed[::2, :] = 1.0
+    assert np.allclose(co.bed_std(bed, F), 0.5)
+    assert np.allclose(co.coarse_bed(bed, F), 0.5)
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(5)
+    N, H, W = 2, 5, 6
+    feat = rng.normal(0, 1, (N, H, W, co.FEAT))
+    qp = rng.normal(0, 1, (N, H, W, 2))
+    qt = qp * (1 + rng.normal(0, 0.05, qp.shape))
+    mask = (rng.uniform(0, 1, qp.shape) > 0.2).astype(np.uint8)
+    p = rand_params(rng)
+    loss, g = co.loss_and_grad(feat, p, qp, qt, mask, emulate=False)
+    live = np.nonzero(co.live_mask())[0]
+    pick = np.concatenate([rng.choice(live, 40, replace=False),
+                           [co.LP - 1 - co.C + 5, 3 * co.LP + co.TAPS * co.C * co.C + 1]])  # two biases
+    eps = 1e-6
+    for i in pick:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        lp, _ = co.loss_and_grad(feat, pp, qp, qt, mask, emulate=False)
+        lm, _ = co.loss_and_grad(feat, pm, qp, qt, mask, emulate=False)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+    # padded parameters get no gradient
+    assert np.all(g[~co.live_mask()] == 0)
+
+
+def test_emulated_gradient_tracks_exact():
+    rng = np.random.default_rng(6)
+    N, H, W = 2, 12, 10
+    feat = rng.normal(0, 1, (N, H, W, co.FEAT))
+    qp = rng.normal(0, 1, (N, H, W, 2))
+    qt = qp * 1.03
+    mask = np.ones(qp.shape, np.uint8)
+    p = rand_params(rng, 0.2)
+    _, g64 = co.loss_and_grad(feat, p, qp, qt, mask, emulate=False)
+    _, g16 = co.loss_and_grad(feat, p, qp, qt, mask, emulate=True)
+    cos = float(g64 @ g16 / (np.linalg.norm(g64) * np.linalg.norm(g16)))
+    assert cos > 0.995, cos
+
+
+def test_gelu_derivative():
+    x = np.linspace(-4, 4, 101)
+    fd = (co.gelu(x + 1e-6) - co.gelu(x - 1e-6)) / 2e-6
+    assert np.abs(fd - co.dgelu(x)).max() < 1e-7
+    assert abs(co.gelu(0.0)) == 0.0 and abs(co.gelu(10.0) - 10.0) < 1e-6
+
+
+def test_adam_first_step_is_signed_lr():
+    rng = np.random.default_rng(1)
+    w = np.where(co.live_mask(), rng.normal(0, 1, co.NPARAMS), 0).astype(np.float32)
+    g = np.where(co.live_mask(), rng.normal(0, 1, co.NPARAMS), 0).astype(np.float32)
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    w1, m1, v1 = co.adam(w, m, v, g, 1, 1e-3)
+    live = co.live_mask() & (np.abs(g) > 1e-3)
+    step = (w - w1)[live]
+    assert np.allclose(step, 1e-3 * np.sign(g[live]), rtol=1e-4)
+    assert np.all(w1[~co.live_mask()] == 0)
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7 + 2 ** -8, 3.14159, -0.0], np.float32)
+    r = co.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0  # tie to even
+    assert r[2] == np.float32(1.0 + 2 ** -6)  # tie to even (up)
+    assert abs(r[3] - 3.140625) < 1e-7
+
+
+def test_metrics():
+    a = np.zeros((4, 4))
+    b = np.zeros((4, 4))
+    b[0, 0] = 0.2
+    assert abs(co.depth_rmse(a, b) - np.sqrt(0.04 / 16)) < 1e-12
+    assert co.max_depth_error(a, b) == pytest.approx(0.2)
+    a[0, 0] = 0.5
+    a[1, 1] = 0.5
+    assert co.wet_iou(a, b) == pytest.approx(0.5)
+    assert co.interior_face_mask(8, 8)[:, :, 0].sum() == 4 * 5

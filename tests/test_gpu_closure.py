@@ -1,0 +1,183 @@
+"""The learned closure on the GPU against its oracle: the matrix-core operand layout, the
+convolution forward and backward passes, the restriction, Adam training, evaluation and
+the inference hook (corrections, saturation fallback, mass balance)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from fip_amd.workloads import BCSpec, Case, SplitMix64, seed_for, terrain_plane_hills
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import closure_oracle as co  # noqa: E402
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+
+def bf16_bits(x):
+    return (co.bf16(x).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def rand_params(rng, scale=0.2):
+    return np.where(co.live_mask(), rng.normal(0.0, scale, co.NPARAMS), 0.0).astype(np.float32)
+
+
+def test_mfma_operand_layout():
+    from fip_amd.sim import mfma_probe
+    rng = np.random.default_rng(0)
+    A = co.bf16(rng.normal(0, 1, (16, 32)).astype(np.float32))
+    B = co.bf16(rng.normal(0, 1, (32, 16)).astype(np.float32))
+    D = mfma_probe(bf16_bits(A), bf16_bits(B))
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.abs(D - ref).max() < 1e-4 * np.abs(ref).max(), np.abs(D - ref).max()
+
+
+@pytest.mark.parametrize("N,H,W", [(1, 9, 20), (2, 20, 70), (1, 33, 128)])
+def test_forward_matches_oracle(N, H, W):
+    from fip_amd.sim import Closure
+    rng = np.random.default_rng(N * 100 + H)
+    c = Closure(16, 16, 0, seed=3)
+    p = rand_params(rng)
+    c.params = p
+    feat = rng.normal(0, 1, (N, H, W, co.FEAT)).astype(np.float32)
+    got = c.forward(feat)
+    ref, _ = co.forward(feat, p, emulate=True)
+    err = np.abs(got - ref).max()
+    assert err <= 2e-3 * max(1.0, np.abs(ref).max()), err
+    c.close()
+
+
+def test_backward_matches_oracle():
+    from fip_amd.sim import Closure
+    rng = np.random.default_rng(11)
+    N, H, W = 2, 24, 70
+    c = Closure(16, 16, 0, seed=3)
+    p = rand_params(rng)
+    c.params = p
+    feat = rng.normal(0, 1, (N, H, W, co.FEAT)).astype(np.float32)
+    qp = rng.normal(0, 1, (N, H, W, 2)).astype(np.float32)
+    qt = (qp * (1 + rng.normal(0, 0.05, qp.shape))).astype(np.float32)
+    mask = (rng.uniform(0, 1, qp.shape) > 0.1).astype(np.uint8)
+    loss, g = c.backward(feat, qp, qt, mask)
+    rloss, rg = co.loss_and_grad(feat, p, qp, qt, mask, emulate=True)
+    assert abs(loss - rloss) <= 1e-4 * abs(rloss), (loss, rloss)
+    for l in range(co.LAYERS):
+        a = g[l * co.LP:(l + 1) * co.LP].astype(np.float64)
+        b = rg[l * co.LP:(l + 1) * co.LP]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert rel < 2e-2, (l, rel)
+    assert np.all(g[~co.live_mask()] == 0)
+    c.close()
+
+
+def fine_coarse_case(W, H, dx, F, scenario=0, wet=0.35):
+    rng = SplitMix64(seed_for(3, scenario))
+    bed = terrain_plane_hills(W, H, dx, 0.002, max(2, W * H // 16384), rng, sigma=(3.0, 12.0))
+    bed = bed.astype(np.float32)
+    level = np.float32(np.quantile(bed, wet))
+    h = np.maximum(np.float32(0), level - bed).astype(np.float32)
+    fine = Case(name="fine", W=W, H=H, dx=dx, dt=0.05, duration=1e9, bed=bed, h=h, kst=30.0,
+                variable_dt=False,
+                bcs=[BCSpec("close", s) for s in ("left", "right", "top", "bottom")])
+    cb = co.coarse_bed(bed, F)
+    hc, _, _ = co.restrict(h, np.zeros_like(h), np.zeros_like(h), F)
+    coarse = Case(name="coarse", W=W // F, H=H // F, dx=dx * F, dt=0.05 * F, duration=1e9, bed=cb, h=hc,
+                  kst=30.0, variable_dt=False,
+                  bcs=[BCSpec("close", s) for s in ("left", "right", "top", "bottom")])
+    return fine, coarse, co.bed_std(bed, F), float(cb.mean())
+
+
+def test_restriction_matches_oracle():
+    from fip_amd.sim import Sim, restrict
+    fine, coarse, _, _ = fine_coarse_case(256, 192, 1.0, 4)
+    f = Sim(fine, mode="dense")
+    f.set_run(1e9)
+    f.simulate(40)
+    c = Sim(coarse, mode="dense")
+    restrict(f, c, 4)
+    hc, qxc, qyc = co.restrict(f.field("h"), f.field("qx"), f.field("qy"), 4)
+    for side in (0, 1):
+        assert np.array_equal(c.field("h", side).view(np.uint32), hc.view(np.uint32))
+        assert np.array_equal(c.field("qx", side).view(np.uint32), qxc.view(np.uint32))
+        assert np.array_equal(c.field("qy", side).view(np.uint32), qyc.view(np.uint32))
+    assert c.time()[0] == f.time()[0]
+    f.close()
+    c.close()
+
+
+def make_samples(cl, nsamp, F=4, W=256, H=256, scenario=0, fine_steps=4):
+    """Samples from one fine run: restrict, one plain coarse step, 4 fine steps, store."""
+    from fip_amd.sim import Sim, restrict
+    fine, coarse, bstd, zref = fine_coarse_case(W, H, 1.0, F, scenario)
+    f = Sim(fine, mode="dense")
+    f.set_run(1e9)
+    c = Sim(coarse, mode="dense")
+    for k in range(nsamp):
+        f.simulate(60)
+        restrict(f, c, F)
+        c.compute_step()
+        c.bc_apply()
+        f.simulate(fine_steps)
+        cl.add_sample(c, f, bstd, zref, F)
+    f.close()
+    c.close()
+    return bstd, zref
+
+
+def test_training_reduces_loss_and_beats_plain():
+    from fip_amd.sim import Closure
+    cl = Closure(64, 64, max_samples=16, seed=5)
+    make_samples(cl, 12, scenario=1)
+    make_samples(cl, 4, scenario=2)
+    assert cl.num_samples() == 16
+    p0 = cl.params
+    plain, learned0 = cl.eval(12, 4)
+    cl.params = np.zeros_like(p0)
+    plain2, zero = cl.eval(12, 4)
+    assert zero == plain2 == plain  # no correction: the plain step
+    cl.params = p0
+    losses = cl.train(150, batch=8, patch=32, lr=2e-3, first=0, n=12)
+    assert np.isfinite(losses).all()
+    assert losses[-20:].mean() < losses[:10].mean()
+    plain3, learned = cl.eval(12, 4)
+    print(f"held-out mse plain {plain3:.3e} learned {learned:.3e}", flush=True)
+    assert learned < plain3
+    cl.close()
+
+
+def test_inference_hook_mass_and_fallback():
+    from fip_amd.sim import Closure, Sim
+    fine, coarse, bstd, zref = fine_coarse_case(256, 256, 1.0, 4, scenario=4, wet=0.5)
+    cl = Closure(64, 64, 0, seed=9)
+    rng = np.random.default_rng(2)
+    cl.params = rand_params(rng, 0.3)
+    plain = Sim(coarse, mode="dense")
+    plain.set_run(1e9)
+    plain.simulate(150)
+    c = Sim(coarse, mode="dense")
+    cl.attach(c, bstd, zref, sat=50.0, mass_limit=1e-4)
+    c.set_run(1e9)
+    c.simulate(150)
+    st = cl.status(c)
+    assert st["fallback"] == 0 and st["max_rel_mass_error"] < 1e-5, st
+    assert not np.array_equal(c.field("qx"), plain.field("qx"))
+    v0 = coarse.h.astype(np.float64)[1:-1, 1:-1].sum()
+    v1 = c.field("h").astype(np.float64)[1:-1, 1:-1].sum()
+    assert abs(v1 - v0) / v0 < 1e-4  # closed box: volume kept
+    # saturation: every face keeps the plain step
+    big = Closure(64, 64, 0, seed=9)
+    big.params = rand_params(rng, 0.3) * 1000
+    d = Sim(coarse, mode="dense")
+    big.attach(d, bstd, zref, sat=1e-6)
+    d.set_run(1e9)
+    d.simulate(150)
+    assert big.status(d)["saturated_faces"] > 0
+    assert np.array_equal(d.field("qx").view(np.uint32), plain.field("qx").view(np.uint32))
+    Closure.detach(c)
+    for x in (plain, c, d):
+        x.close()
+    cl.close()
+    big.close()
