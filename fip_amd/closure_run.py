@@ -1,0 +1,197 @@
+"""Training data, training and evaluation of the learned subgrid closure, driven through
+the C ABI (include/fip_closure.h); every step runs on the GPU.
+
+Fine scenarios (dx = 1 m) follow the config-0 recipe with 1 m-scale ridges: a plane
+sloping 0.002, Gaussian hills with sigma ~ U(3, 20) cells, a 20-cell channel 3 m deep
+along y = H/2, `nridges` straight ridges 2 cells wide and U(0.3, 1.5) m high, uniform
+kSt ~ U(20, 45); left type-z inflow over the channel rows (z = bed + 2 m), right open, top
+and bottom closed; dry start. The coarse twin is the 4x restriction (bed = block mean,
+z BC levels = the mean of the four fine levels). Both run with a fixed dt (0.05 s fine,
+0.2 s coarse: one coarse step = four fine steps, the pairing the closure is trained on).
+
+Training pairs: every `sample_every` seconds of a fine run, its state is restricted onto
+the coarse grid, the coarse solver takes one plain step, the fine solver four steps, and
+(restricted input, plain coarse fluxes, restricted fine fluxes) is stored.
+Evaluation: held-out scenarios run plain fine, plain coarse and learned coarse (closure
+after every coarse step) to the same time; the coarse results are judged against the
+restricted fine run (depth RMSE, max-depth error, wet-extent IoU).
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from .workloads import BCSpec, Case, SplitMix64, add_hills, seed_for
+
+F = 4
+DT_FINE = 0.05
+
+
+def _ridges(H: int, W: int, rng: SplitMix64, n: int) -> np.ndarray:
+    """n straight ridges 2 cells wide, U(0.3, 1.5) m high, random position, direction and
+    length (5 to 30 % of the grid); overlapping ridges keep the higher crest."""
+    ridge = np.zeros((H, W), np.float64)
+    for _ in range(n):
+        x0, y0 = rng.uniform(0, W), rng.uniform(0, H)
+        ang = rng.uniform(0, math.pi)
+        length = rng.uniform(0.05, 0.3) * min(W, H)
+        height = rng.uniform(0.3, 1.5)
+        steps = int(length * 2) + 1
+        t = np.linspace(-0.5, 0.5, steps)
+        xs = (x0 + t * length * math.cos(ang)).astype(np.int64)
+        ys = (y0 + t * length * math.sin(ang)).astype(np.int64)
+        for dy in (0, 1):
+            for dx in (0, 1):
+                yy, xx = ys + dy, xs + dx
+                ok = (yy >= 0) & (yy < H) & (xx >= 0) & (xx < W)
+                ridge[yy[ok], xx[ok]] = np.maximum(ridge[yy[ok], xx[ok]], height)
+    return ridge
+
+
+def fine_case(scenario: int, W: int = 1024, H: int = 1024, nridges: int = 200, duration: float = 3600.0,
+              seed_config: int = 3) -> Case:
+    rng = SplitMix64(seed_for(seed_config, scenario))
+    dx = 1.0
+    x = np.arange(W, dtype=np.float64)
+    bed = np.broadcast_to(0.002 * dx * (W - 1 - x), (H, W)).copy()
+    nh = max(4, int(round(64 * W * H / (512.0 * 512.0))))
+    add_hills(bed, rng, nh, (0.0, 5.0), (3.0, 20.0))
+    c = H // 2
+    bed[c - 10:c + 10, :] -= 3.0
+    bed += _ridges(H, W, rng, nridges)
+    kst = float(np.float32(rng.uniform(20.0, 45.0)))
+    bed32 = bed.astype(np.float32)
+    bcs = [BCSpec("z", "left", row, row, z=float(bed32[row, 0] + np.float32(2.0)))
+           for row in range(c - 20, c + 20)]
+    bcs += [BCSpec("open", "right"), BCSpec("close", "top"), BCSpec("close", "bottom")]
+    return Case(name=f"fine_{scenario}", W=W, H=H, dx=dx, dt=DT_FINE, duration=duration, bed=bed32,
+                kst=kst, variable_dt=False, bcs=bcs)
+
+
+def coarse_of(fine: Case) -> tuple[Case, np.ndarray, float]:
+    """The 4x coarse twin, the subgrid bed std and the feature reference level."""
+    W, H = fine.W // F, fine.H // F
+    b = fine.bed.astype(np.float64).reshape(H, F, W, F)
+    cb = (b.sum(axis=(1, 3)) / (F * F)).astype(np.float32)
+    bstd = np.sqrt(((b - b.mean(axis=(1, 3), keepdims=True)) ** 2).mean(axis=(1, 3))).astype(np.float32)
+    zl = {}
+    for bc in fine.bcs:
+        if bc.type == "z":
+            zl.setdefault(bc.frm // F, []).append(bc.z)
+    bcs = [BCSpec("z", "left", r, r, z=float(np.float32(np.mean(v)))) for r, v in sorted(zl.items())]
+    bcs += [BCSpec("open", "right"), BCSpec("close", "top"), BCSpec("close", "bottom")]
+    coarse = Case(name=fine.name + "_c", W=W, H=H, dx=fine.dx * F, dt=fine.dt * F, duration=fine.duration,
+                  bed=cb, kst=fine.kst, variable_dt=False, bcs=bcs)
+    return coarse, bstd, float(cb.mean())
+
+
+def block_mean(a: np.ndarray) -> np.ndarray:
+    H, W = a.shape
+    return (a.astype(np.float64).reshape(H // F, F, W // F, F).sum(axis=(1, 3)) / (F * F)).astype(np.float32)
+
+
+def generate_samples(cl, scenarios, duration, sample_every=60.0, variant="fast", W=1024, log=print):
+    """Adds (duration / sample_every) samples per scenario to the closure's store."""
+    from .sim import Sim, restrict
+    per = int(round(sample_every / DT_FINE)) - F
+    t0 = time.perf_counter()
+    for s in scenarios:
+        fine = fine_case(s, W=W, H=W, duration=duration)
+        coarse, bstd, zref = coarse_of(fine)
+        f = Sim(fine, mode="dense", variant=variant)
+        f.set_run(1e12)
+        c = Sim(coarse, mode="dense", variant=variant)
+        for _ in range(int(duration // sample_every)):
+            f.simulate(per)
+            restrict(f, c, F)
+            c.compute_step()
+            c.bc_apply()
+            f.simulate(F)
+            cl.add_sample(c, f, bstd, zref, F)
+        f.close()
+        c.close()
+    log(f"samples: {cl.num_samples()} in {time.perf_counter() - t0:.1f} s")
+
+
+def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print):
+    """Plain fine vs plain coarse vs learned coarse on held-out scenarios, as ensembles.
+    Returns metrics and timings."""
+    from . import metrics as co
+    from .sim import Ensemble
+    fines = [fine_case(s, W=W, H=W, duration=duration) for s in scenarios]
+    pairs = [coarse_of(f) for f in fines]
+    coarses = [p[0] for p in pairs]
+    bstd = np.stack([p[1] for p in pairs])
+    zref = np.array([p[2] for p in pairs], np.float32)
+    out = {"scenarios": len(scenarios), "duration_s": duration, "fine": f"{fines[0].W}x{fines[0].H} dx=1",
+           "coarse": f"{coarses[0].W}x{coarses[0].H} dx=4"}
+
+    def run(cases, closure):
+        e = Ensemble(cases, mode="dense", variant=variant, tile_rows=32)
+        if closure is not None:
+            closure.attach_ensemble(e, bstd, zref)
+        e.set_run([duration] * len(cases))
+        e.simulate(200)  # graphs built, warm
+        t0 = time.perf_counter()
+        e.simulate(10 ** 9)
+        wall = time.perf_counter() - t0
+        return e, wall
+
+    ef, wf = run(fines, None)
+    ep, wp = run(coarses, None)
+    el, wl = run(coarses, cl)
+    rm, me, iou = {"plain": [], "learned": []}, {"plain": [], "learned": []}, {"plain": [], "learned": []}
+    fb = 0
+    mass = 0.0
+    for k in range(len(scenarios)):
+        href = block_mean(ef.field(k, "h"))
+        hmref = block_mean(ef.field(k, "hmax"))
+        for name, e in (("plain", ep), ("learned", el)):
+            h = e.field(k, "h")
+            rm[name].append(co.depth_rmse(h, href))
+            me[name].append(co.max_depth_error(e.field(k, "hmax"), hmref))
+            iou[name].append(co.wet_iou(h, href))
+        st = cl.status(ens=el, s=k)
+        fb += st["fallback"]
+        mass = max(mass, st["max_rel_mass_error"])
+    sim_h = len(scenarios) * (duration - 0.0) / 3600.0
+    for name, wall in (("fine", wf), ("plain_coarse", wp), ("learned_coarse", wl)):
+        out[name + "_sim_h_per_wall_s"] = round(sim_h / wall, 3)
+        out[name + "_wall_s"] = round(wall, 2)
+    out["speedup_learned_vs_fine"] = round(wf / wl, 2)
+    for name in ("plain", "learned"):
+        out[f"{name}_depth_rmse_m"] = round(float(np.mean(rm[name])), 4)
+        out[f"{name}_max_depth_err_m"] = round(float(np.mean(me[name])), 4)
+        out[f"{name}_wet_iou"] = round(float(np.mean(iou[name])), 4)
+    out["fallback_scenarios"] = fb
+    out["max_rel_mass_error"] = mass
+    for e in (ef, ep, el):
+        e.close()
+    log(str(out))
+    return out
+
+
+def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1500, batch=32, patch=128,
+               lr=1e-3, W=1024, log=print):
+    from .sim import Closure
+    Wc = W // F
+    nsamp = n_train * int(duration // sample_every) + n_test * 4
+    cl = Closure(Wc, Wc, max_samples=nsamp, seed=11)
+    generate_samples(cl, range(n_train), duration, sample_every, W=W, log=log)
+    ntr = cl.num_samples()
+    generate_samples(cl, range(1000, 1000 + n_test), 4 * sample_every, sample_every, W=W, log=log)
+    plain0, learned0 = cl.eval(ntr, cl.num_samples() - ntr)
+    t0 = time.perf_counter()
+    losses = cl.train(steps, batch=batch, patch=patch, lr=lr, first=0, n=ntr)
+    twall = time.perf_counter() - t0
+    plain1, learned1 = cl.eval(ntr, cl.num_samples() - ntr)
+    res = {"train_samples": ntr, "train_steps": steps, "train_wall_s": round(twall, 2),
+           "loss_first10": float(np.mean(losses[:10])), "loss_last50": float(np.mean(losses[-50:])),
+           "heldout_flux_mse_plain": plain1, "heldout_flux_mse_learned": learned1,
+           "heldout_flux_mse_untrained": learned0}
+    log(str(res))
+    res["eval"] = evaluate(cl, range(1000, 1000 + n_test), duration, W=W, log=log)
+    cl.close()
+    return res

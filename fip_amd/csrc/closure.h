@@ -29,8 +29,8 @@ struct ClFeat {
 };
 
 struct ClMass {
-  double v_prev;     // interior volume after the previous step
-  double phi;        // volume the held boundary faces bring in during the next step
+  double v_prev;     // volume of the strict interior after the previous step
+  double t_prev;     // duration after the previous step
   double max_rel;    // largest relative mass error seen
   double last_rel;
   int valid;
@@ -40,6 +40,19 @@ struct ClMass {
 };
 
 
+// One coarse domain of the fused inference step.
+struct ClDom {
+  fip_grid g;
+  const float* bstd;   // W x H dense
+  float zref;
+  int pad;
+};
+#define CL_PACKED_L1 (2 * 3 * 64 * 8)   // bf16 elements of the 8-channel packing of layer 0
+
+hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
+                           const float* w, float bound, float sat, ClMass* ms, double* acc, float* o_out,
+                           hipStream_t st);
+hipError_t cl_pack_l1_launch(const float* w, u16* wl1, hipStream_t st);
 hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
                           u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,
                           hipStream_t st);

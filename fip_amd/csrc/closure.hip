@@ -438,59 +438,74 @@ extern "C" __global__ void cl_correct(fip_grid g, int par, const float* o, float
   if (nsat) atomicAdd(&ms->nsat, nsat);
 }
 
-// Mass balance of a coarse domain without no_data cells, after a step and its
-// correction. The continuity update of the stored cells [1, W-2] x [1, H-2] telescopes, so
-// their volume changes in the next step by exactly what the held faces (qx at columns 0
-// and W-2, qy at rows 0 and H-2) bring in: phi = -dt dx (sum_j qx(W-2,j) - qx(0,j) +
-// sum_i qy(i,H-2) - qy(i,0)). cl_mass_sum adds the volume and phi of the current state
-// into acc[2] (fp64); cl_mass_check compares the volume with the previous prediction and
-// sets the fallback flag when the relative error exceeds `limit`.
-extern "C" __global__ void cl_mass_sum(fip_grid g, int par, double* acc) {
-  if (g.ctl->halt_req[par ^ 1]) return;
-  const int cur = par ^ 1;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  double v = 0.0, f = 0.0;
-  if (x >= 1 && x <= g.W - 2 && y >= 1 && y <= g.H - 2) v = (double)g.h[cur][(long)y * g.P + x];
-  if (x == 0 && y >= 1 && y <= g.H - 2)
-    f = (double)g.qx[cur][(long)y * g.P + g.W - 2] - (double)g.qx[cur][(long)y * g.P];
-  if (y == 0 && x >= 1 && x <= g.W - 2)
-    f += (double)g.qy[cur][(long)(g.H - 2) * g.P + x] - (double)g.qy[cur][x];
-  __shared__ double sh[2][256];
-  sh[0][threadIdx.x] = v;
-  sh[1][threadIdx.x] = f;
+// Mass balance of a coarse domain without no_data cells, checked after each step before
+// its correction. The continuity update telescopes over the box of cells
+// [2, W-3] x [2, H-3], which no boundary condition writes, so over the step just taken
+//   V(t) - V(t - dt) = -dt dx (sum_j qx(W-3, j) - qx(1, j) + sum_i qy(i, H-3) - qy(i, 1))
+// with the face discharges the step computed (stored and not yet corrected). cl_mass_sum
+// adds the box volume and the bracket into acc[2] (fp64); cl_mass_check compares and sets
+// the fallback flag when the relative error exceeds `limit`.
+__device__ __forceinline__ void mass_terms(const fip_grid& g, int cur, int x, int y, double& v, double& f) {
+  v = 0.0;
+  f = 0.0;
+  if (x >= 2 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) v = (double)g.h[cur][(long)y * g.P + x];
+  if (x == 0 && y >= 2 && y <= g.H - 3)
+    f = (double)g.qx[cur][(long)y * g.P + g.W - 3] - (double)g.qx[cur][(long)y * g.P + 1];
+  if (y == 0 && x >= 2 && x <= g.W - 3)
+    f += (double)g.qy[cur][(long)(g.H - 3) * g.P + x] - (double)g.qy[cur][(long)g.P + x];
+}
+
+// sh: 2 x blockDim.x doubles of LDS provided by the kernel (blockDim.x a power of 2)
+__device__ __forceinline__ void block_sum2(double v, double f, double* acc, double* sh) {
+  const int n = blockDim.x, t = threadIdx.x;
+  sh[t] = v;
+  sh[n + t] = f;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+  for (int w = n / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[t] += sh[t + w];
+      sh[n + t] += sh[n + t + w];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    atomicAdd(&acc[0], sh[0][0]);
-    atomicAdd(&acc[1], sh[1][0]);
+  if (t == 0) {
+    atomicAdd(&acc[0], sh[0]);
+    atomicAdd(&acc[1], sh[n]);
   }
+}
+
+__device__ __forceinline__ void mass_check(const fip_grid& g, double* acc, ClMass& m, double limit) {
+  const double dx = g.dx;
+  const double vol = acc[0] * dx * dx;
+  const double t = g.ctl->duration;
+  if (m.valid) {
+    const double pred = m.v_prev - (t - m.t_prev) * dx * acc[1];
+    const double rel = fabs(vol - pred) / (vol > 1e-30 ? vol : 1e-30);
+    m.last_rel = rel;
+    if (rel > m.max_rel) m.max_rel = rel;
+    if (rel > limit && vol > 0.0) m.fallback = 1;
+  }
+  m.v_prev = vol;
+  m.t_prev = t;
+  m.valid = 1;
+  m.steps = m.steps + 1;
+  acc[0] = 0.0;
+  acc[1] = 0.0;
+}
+
+extern "C" __global__ void cl_mass_sum(fip_grid g, int par, double* acc) {
+  if (g.ctl->halt_req[par ^ 1]) return;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  double v, f;
+  mass_terms(g, par ^ 1, x, y, v, f);
+  __shared__ double sh[2 * 256];
+  block_sum2(v, f, acc, sh);
 }
 
 extern "C" __global__ void cl_mass_check(fip_grid g, int par, double* acc, ClMass* ms, double limit) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (g.ctl->halt_req[par ^ 1]) return;
-  const double dx = g.dx;
-  const double vol = acc[0] * dx * dx;
-  const double phi = -(double)g.ctl->dt * dx * acc[1];
-  if (ms->valid) {
-    const double pred = ms->v_prev + ms->phi;
-    const double rel = fabs(vol - pred) / (vol > 1e-30 ? vol : 1e-30);
-    ms->last_rel = rel;
-    if (rel > ms->max_rel) ms->max_rel = rel;
-    if (rel > limit && vol > 0.0) ms->fallback = 1;
-  }
-  ms->v_prev = vol;
-  ms->phi = phi;
-  ms->valid = 1;
-  ms->steps = ms->steps + 1;
-  acc[0] = 0.0;
-  acc[1] = 0.0;
+  mass_check(g, acc, *ms, limit);
 }
 
 // Ensemble forms of cl_correct / cl_mass_sum / cl_mass_check: blockIdx.z (or the thread
@@ -525,29 +540,11 @@ extern "C" __global__ void cl_mass_sum_ens(const fip_ens_scn* scn, int par, doub
   const int s = blockIdx.z;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
-  const int cur = par ^ 1;
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  double v = 0.0, f = 0.0;
-  if (x >= 1 && x <= g.W - 2 && y >= 1 && y <= g.H - 2) v = (double)g.h[cur][(long)y * g.P + x];
-  if (x == 0 && y >= 1 && y <= g.H - 2)
-    f = (double)g.qx[cur][(long)y * g.P + g.W - 2] - (double)g.qx[cur][(long)y * g.P];
-  if (y == 0 && x >= 1 && x <= g.W - 2)
-    f += (double)g.qy[cur][(long)(g.H - 2) * g.P + x] - (double)g.qy[cur][x];
-  __shared__ double sh[2][256];
-  sh[0][threadIdx.x] = v;
-  sh[1][threadIdx.x] = f;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    atomicAdd(&acc[2 * s], sh[0][0]);
-    atomicAdd(&acc[2 * s + 1], sh[1][0]);
-  }
+  double v, f;
+  mass_terms(g, par ^ 1, x, y, v, f);
+  __shared__ double sh[2 * 256];
+  block_sum2(v, f, acc + 2 * s, sh);
 }
 
 extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms,
@@ -556,23 +553,195 @@ extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int 
   if (s >= n) return;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
-  const double dx = g.dx;
-  const double vol = acc[2 * s] * dx * dx;
-  const double phi = -(double)g.ctl->dt * dx * acc[2 * s + 1];
-  ClMass& m = ms[s];
-  if (m.valid) {
-    const double pred = m.v_prev + m.phi;
-    const double rel = fabs(vol - pred) / (vol > 1e-30 ? vol : 1e-30);
-    m.last_rel = rel;
-    if (rel > m.max_rel) m.max_rel = rel;
-    if (rel > limit && vol > 0.0) m.fallback = 1;
+  mass_check(g, acc + 2 * s, ms[s], limit);
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused inference step of the closure for 16 x 16 coarse cells per work-group: features,
+// the four convolutions (activations kept in LDS, the halo recomputed: layer l is
+// evaluated on (16 + 2 (4 - l))^2 cells), the bounded correction of the cells' east/north
+// face discharges and the mass-balance sums, in one launch. blockIdx.z = domain.
+// Activations outside the domain are 0, as the zero padding of the layer-by-layer form.
+// The mass sums use the face discharges before correction (each face is owned by one
+// cell, so no work-group reads a face another one corrects); a fallback raised by the
+// mass check after this step takes effect from the next step.
+#define FT 16                 // output tile
+#define FR0 (FT + 8)          // feature region
+#define FR1 (FT + 6)
+#define FR2 (FT + 4)
+#define FR3 (FT + 2)
+#define FPS 40                // LDS pixel stride of 32-channel activations (u16): conflict-free
+// 8-channel packing of layer 0: K chunk c holds taps 4c .. 4c+3 (lane group = tap - 4c),
+// 8 channels each; taps 9..11 are zero.
+__device__ __forceinline__ long cl_pack_l1_index(int tap, int ci, int co) {
+  const int mt = co / 16, chunk = tap / 4, lane = (co % 16) + 16 * (tap % 4), j = ci;
+  return (((long)mt * 3 + chunk) * 64 + lane) * 8 + j;
+}
+
+extern "C" __global__ void cl_pack_l1(const float* w, u16* wl1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= CL_TAPS * 8 * CL_C) return;
+  const int tap = i / (8 * CL_C), ci = (i / CL_C) % 8, co = i % CL_C;
+  wl1[cl_pack_l1_index(tap, ci, co)] = f2bf(w[((long)tap * CL_C + ci) * CL_C + co]);
+}
+
+#define FNW 8   // wavefronts per work-group of cl_fused
+
+// One layer of the fused step: wavefront w owns pixel tiles w, w + FNW, ... of the output
+// region (16 pixels each, row-major over the region); for each K chunk it loads the A
+// fragments once and multiplies them into all of its tiles.
+template <int RIN, int ROUT, int MT, bool L1>
+__device__ __forceinline__ void fused_layer(const u16* in, u16* out, const u16* wp, const float* bias,
+                                            int ox, int oy, int W, int H, float* o2) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  constexpr int NPIX = ROUT * ROUT, NT = (NPIX + 15) / 16, TPW = (NT + FNW - 1) / FNW;
+  constexpr int off = (RIN - ROUT) / 2;   // 1
+  constexpr int NC = L1 ? 3 : CL_TAPS;    // K chunks of 32
+  const int gx0 = ox - (ROUT - FT) / 2, gy0 = oy - (ROUT - FT) / 2;
+  f32x4 acc[TPW][MT];
+  int base[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; i++) {
+#pragma unroll
+    for (int m = 0; m < MT; m++) acc[i][m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int p = (wave + FNW * i) * 16 + col;
+    const int pp = p < NPIX ? p : NPIX - 1;
+    base[i] = (pp / ROUT + off) * RIN + pp % ROUT + off;   // input pixel of tap (0, 0)
   }
-  m.v_prev = vol;
-  m.phi = phi;
-  m.valid = 1;
-  m.steps = m.steps + 1;
-  acc[2 * s] = 0.0;
-  acc[2 * s + 1] = 0.0;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    bf16x8 a[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) a[m] = ld8(wp + ((m * NC + c) * 64 + lane) * 8);
+#pragma unroll
+    for (int i = 0; i < TPW; i++) {
+      if (wave + FNW * i >= NT) break;
+      bf16x8 b;
+      if constexpr (L1) {
+        const int tap = 4 * c + grp;
+        const int sh = tap < CL_TAPS ? (tap / 3 - 1) * RIN + (tap % 3 - 1) : 0;
+        b = tap < CL_TAPS ? ld8(in + (base[i] + sh) * 8) : zero8();
+      } else {
+        const int sh = (c / 3 - 1) * RIN + (c % 3 - 1);
+        b = ld8(in + (base[i] + sh) * FPS + 8 * grp);
+      }
+#pragma unroll
+      for (int m = 0; m < MT; m++) acc[i][m] = mfma(a[m], b, acc[i][m]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TPW; i++) {
+    const int p = (wave + FNW * i) * 16 + col;
+    if (p >= NPIX) continue;
+    const int yy = p / ROUT, xx = p % ROUT;
+    const bool inside = gx0 + xx >= 0 && gx0 + xx < W && gy0 + yy >= 0 && gy0 + yy < H;
+    if (o2) {  // last layer: outputs 0 and 1, fp32, held by lane group 0
+      if (grp == 0) {
+        o2[p * 2] = acc[i][0][0] + bias[0];
+        o2[p * 2 + 1] = acc[i][0][1] + bias[1];
+      }
+      continue;
+    }
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      const int co0 = m * 16 + 4 * grp;
+      uint32_t lo, hi;
+      {
+        const u16 z0 = f2bf(acc[i][m][0] + bias[co0]), z1 = f2bf(acc[i][m][1] + bias[co0 + 1]);
+        const u16 z2 = f2bf(acc[i][m][2] + bias[co0 + 2]), z3 = f2bf(acc[i][m][3] + bias[co0 + 3]);
+        const u16 a0 = inside ? f2bf(gelu(bf2f(z0))) : (u16)0, a1 = inside ? f2bf(gelu(bf2f(z1))) : (u16)0;
+        const u16 a2 = inside ? f2bf(gelu(bf2f(z2))) : (u16)0, a3 = inside ? f2bf(gelu(bf2f(z3))) : (u16)0;
+        lo = (uint32_t)a0 | ((uint32_t)a1 << 16);
+        hi = (uint32_t)a2 | ((uint32_t)a3 << 16);
+      }
+      *(uint2*)(out + (yy * ROUT + xx) * FPS + co0) = make_uint2(lo, hi);
+    }
+  }
+}
+
+extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, int par, const u16* wf,
+                                                          const u16* wl1, const float* w, float bound,
+                                                          float sat, ClMass* ms, double* acc, float* o_out) {
+  __shared__ __attribute__((aligned(16))) u16 f0[FR0 * FR0 * 8];
+  __shared__ __attribute__((aligned(16))) u16 b1[FR1 * FR1 * FPS];
+  __shared__ __attribute__((aligned(16))) u16 b2[FR2 * FR2 * FPS];
+  const int d = blockIdx.z;
+  const fip_grid& g = dom[d].g;
+  if (g.ctl->halt_req[par ^ 1]) return;
+  const int cur = par ^ 1;
+  const int W = g.W, H = g.H;
+  const int ox = blockIdx.x * FT, oy = blockIdx.y * FT;
+  const float* bstd = dom[d].bstd;
+  const float zref = dom[d].zref;
+  // features of the 24 x 24 region (the formulas of feat_pixel)
+  for (int e = threadIdx.x; e < FR0 * FR0; e += blockDim.x) {
+    const int gx = ox - 4 + e % FR0, gy = oy - 4 + e / FR0;
+    uint4 pk = make_uint4(0, 0, 0, 0);
+    if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+      const long o = (long)gy * g.P + gx;
+      const float b = g.bed[o];
+      const float f0v = (g.h[cur][o] + b - zref) * 0.25f;
+      const float f1v = g.qx[cur][o], f2v = g.qy[cur][o];
+      const float f3v = (b - zref) * 0.25f;
+      const float f4v = bstd[(long)gy * W + gx] * 2.0f;
+      const float f5v = (g.kst ? g.kst[o] : g.kst_uniform) * (1.0f / 30.0f);
+      pk.x = (uint32_t)f2bf(f0v) | ((uint32_t)f2bf(f1v) << 16);
+      pk.y = (uint32_t)f2bf(f2v) | ((uint32_t)f2bf(f3v) << 16);
+      pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
+    }
+    *(uint4*)(f0 + e * 8) = pk;
+  }
+  __syncthreads();
+  const float* b0 = w + 0 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C;
+  fused_layer<FR0, FR1, 2, true>(f0, b1, wl1, b0, ox, oy, W, H, nullptr);
+  __syncthreads();
+  fused_layer<FR1, FR2, 2, false>(b1, b2, wf + 1 * CL_PACKED, w + 1 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C, ox,
+                                  oy, W, H, nullptr);
+  __syncthreads();
+  fused_layer<FR2, FR3, 2, false>(b2, b1, wf + 2 * CL_PACKED, w + 2 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C, ox,
+                                  oy, W, H, nullptr);
+  __syncthreads();
+  float* o2 = (float*)b2;  // 16 x 16 x 2 outputs reuse the layer-2 buffer
+  fused_layer<FR3, FT, 1, false>(b1, nullptr, wf + 3 * CL_PACKED, w + 3 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C,
+                                 ox, oy, W, H, o2);
+  __syncthreads();
+  if (o_out) {  // network outputs only ([H][W][2]), no correction
+    for (int e = threadIdx.x; e < FT * FT; e += blockDim.x) {
+      const int x = ox + e % FT, y = oy + e / FT;
+      if (x < W && y < H) {
+        o_out[((long)y * W + x) * 2] = o2[e * 2];
+        o_out[((long)y * W + x) * 2 + 1] = o2[e * 2 + 1];
+      }
+    }
+    return;
+  }
+  // correction of the tile's faces and the mass-balance sums (pre-correction values)
+  const bool off = ms[d].fallback != 0;
+  double v = 0.0, fl = 0.0;
+  unsigned nsat = 0;
+  for (int e = threadIdx.x; e < FT * FT; e += blockDim.x) {
+    const int x = ox + e % FT, y = oy + e / FT;
+    if (x >= W || y >= H) continue;
+    const long i = (long)y * g.P + x;
+    double mv, mf;
+    mass_terms(g, cur, x, y, mv, mf);
+    v += mv;
+    fl += mf;
+    if (x >= 1 && x <= W - 3 && y >= 2 && y <= H - 3) {
+      const float q = g.qx[cur][i], ov = o2[e * 2];
+      if (fabsf(ov) > sat) nsat++;
+      else if (!off) g.qx[cur][i] = q + tanhf(ov) * bound * fabsf(q);
+    }
+    if (x >= 2 && x <= W - 3 && y >= 1 && y <= H - 3) {
+      const float q = g.qy[cur][i], ov = o2[e * 2 + 1];
+      if (fabsf(ov) > sat) nsat++;
+      else if (!off) g.qy[cur][i] = q + tanhf(ov) * bound * fabsf(q);
+    }
+  }
+  __syncthreads();
+  block_sum2(v, fl, acc + 2 * d, (double*)f0);  // the feature buffer (>= 2 x 512 doubles) is free by now
+  if (nsat) atomicAdd(&ms[d].nsat, nsat);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -718,3 +887,13 @@ CL_WRAP(cl_mass_sum_ens, (const fip_ens_scn* scn, int par, double* acc, int W, i
 CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
                             hipStream_t st),
         P(scn, n, par, acc, ms, limit), dim3((n + 63) / 64), dim3(64))
+CL_WRAP(cl_pack_l1, (const float* w, u16* wl1, hipStream_t st), P(w, wl1), dim3((CL_TAPS * 8 * CL_C + 255) / 256),
+        dim3(256))
+
+hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
+                           const float* w, float bound, float sat, ClMass* ms, double* acc, float* o_out,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(cl_fused, dim3((W + FT - 1) / FT, (H + FT - 1) / FT, ndom), dim3(512), 0, st, dom, par, wf,
+                     wl1, w, bound, sat, ms, acc, o_out);
+  return hipGetLastError();
+}

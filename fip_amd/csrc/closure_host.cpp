@@ -114,14 +114,15 @@ struct Attachment {
   float zref_h = 0.f;
   ClMass* ms = nullptr;     // one per scenario
   double* acc = nullptr;    // 2 per scenario
+  ClDom* dom = nullptr;     // one per scenario
   float sat = 2.65f;
   double limit = 1e-4;
-  Work work;
   ~Attachment() {
     if (bstd) hipFree(bstd);
     if (zref) hipFree(zref);
     if (ms) hipFree(ms);
     if (acc) hipFree(acc);
+    if (dom) hipFree(dom);
   }
 };
 
@@ -136,6 +137,7 @@ struct fip_closure {
   float* g = nullptr;
   u16* wf = nullptr;    // packed forward weights [layer][CL_PACKED]
   u16* wb = nullptr;    // packed backward-data weights
+  u16* wl1 = nullptr;   // layer 0 in the 8-channel packing of the fused inference kernel
   float* part = nullptr;
   int t = 0;            // Adam steps taken
   // training samples
@@ -156,6 +158,7 @@ const float* bias_of(const fip_closure* cl, int l) { return cl->w + (size_t)l * 
 
 int pack(fip_closure* cl) {
   CHIP(cl_pack_launch(cl->w, CL_LAYERS, cl->wf, cl->wb, cl->stream));
+  CHIP(cl_pack_l1_launch(cl->w, cl->wl1, cl->stream));
   return FIP_OK;
 }
 
@@ -235,6 +238,21 @@ std::vector<fip_closure*>& registry() {
 
 }  // namespace
 
+namespace fip_internal {
+void closure_forget(const fip_sim* s, const fip_ens* e) {
+  for (fip_closure* cl : registry())
+    for (auto it = cl->att.begin(); it != cl->att.end();) {
+      if ((s && (*it)->sim == s) || (e && (*it)->ens == e)) {
+        hipSetDevice(cl->dev);
+        hipDeviceSynchronize();
+        it = cl->att.erase(it);
+      } else {
+        ++it;
+      }
+    }
+}
+}  // namespace fip_internal
+
 extern "C" {
 
 int fip_restrict(fip_sim* fine, fip_sim* coarse, int F) {
@@ -273,6 +291,7 @@ int fip_closure_create(int device, unsigned long long seed, int Wc, int Hc, int 
   if (!rc) rc = dalloc(&cl->g, np);
   if (!rc) rc = dalloc(&cl->wf, (size_t)CL_LAYERS * CL_PACKED);
   if (!rc) rc = dalloc(&cl->wb, (size_t)CL_LAYERS * CL_PACKED);
+  if (!rc) rc = dalloc(&cl->wl1, (size_t)CL_PACKED_L1);
   if (!rc) rc = dalloc(&cl->part, (size_t)kWgradBlocks * CL_LAYER_PARAMS);
   const size_t spx = (size_t)Wc * Hc;
   if (!rc) rc = dalloc(&cl->s_feat, (size_t)max_samples * spx * 8);
@@ -282,7 +301,8 @@ int fip_closure_create(int device, unsigned long long seed, int Wc, int Hc, int 
   if (rc) return cleanup(rc);
   if (hipMemset(cl->m, 0, np * sizeof(float)) != hipSuccess || hipMemset(cl->v, 0, np * sizeof(float)) != hipSuccess ||
       hipMemset(cl->wf, 0, (size_t)CL_LAYERS * CL_PACKED * 2) != hipSuccess ||
-      hipMemset(cl->wb, 0, (size_t)CL_LAYERS * CL_PACKED * 2) != hipSuccess)
+      hipMemset(cl->wb, 0, (size_t)CL_LAYERS * CL_PACKED * 2) != hipSuccess ||
+      hipMemset(cl->wl1, 0, (size_t)CL_PACKED_L1 * 2) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipMemset failed"));
   if ((rc = init_params(cl, seed))) return cleanup(rc);
   if (hipStreamSynchronize(cl->stream) != hipSuccess) return cleanup(fail(FIP_E_HIP, "sync failed"));
@@ -300,7 +320,7 @@ int fip_closure_destroy(fip_closure* cl) {
   }
   if (cl->stream) hipStreamSynchronize(cl->stream);
   cl->att.clear();
-  for (void* p : {(void*)cl->w, (void*)cl->m, (void*)cl->v, (void*)cl->g, (void*)cl->wf, (void*)cl->wb,
+  for (void* p : {(void*)cl->w, (void*)cl->m, (void*)cl->v, (void*)cl->g, (void*)cl->wf, (void*)cl->wb, (void*)cl->wl1,
                   (void*)cl->part, (void*)cl->s_feat, (void*)cl->s_qp, (void*)cl->s_qt, (void*)cl->s_mask})
     if (p) hipFree(p);
   cl->train.release();
@@ -447,22 +467,19 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   int rc = dalloc(&a->bstd, spx);
   if (!rc) rc = dalloc(&a->ms, 1);
   if (!rc) rc = dalloc(&a->acc, 2);
-  if (!rc) rc = a->work.ensure(1, g.H, g.W, false);
   if (rc) return rc;
   CHIP(hipMemcpy(a->bstd, bed_std, spx * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemset(a->ms, 0, sizeof(ClMass)));
   CHIP(hipMemset(a->acc, 0, 2 * sizeof(double)));
+  ClDom hd{g, a->bstd, zref, 0};
+  if ((rc = dalloc(&a->dom, 1))) return rc;
+  CHIP(hipMemcpy(a->dom, &hd, sizeof hd, hipMemcpyHostToDevice));
   Attachment* ap = a.get();
   cl->att.push_back(std::move(a));
   fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
-    fip_grid& g = fip_internal::grid_mut(ap->sim);
-    const int cur = par ^ 1;
-    ClFeat fe{g.h[cur], g.qx[cur], g.qy[cur], g.bed, g.kst, ap->bstd, g.kst_uniform, ap->zref_h, g.W, g.H, g.P};
-    CHIP(cl_features_launch(fe, ap->work.x0, CL_C, g.ctl, par, st));
-    int rc = forward(cl, ap->work, g.ctl, par, st);
-    if (rc) return rc;
-    CHIP(cl_correct_launch(g, par, ap->work.o, kBound, ap->sat, ap->ms, st));
-    CHIP(cl_mass_sum_launch(g, par, ap->acc, st));
+    const fip_grid& g = fip_internal::grid(ap->sim);
+    CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, kBound, ap->sat, ap->ms, ap->acc,
+                         nullptr, st));
     CHIP(cl_mass_check_launch(g, par, ap->acc, ap->ms, ap->limit, st));
     return FIP_OK;
   });
@@ -493,24 +510,23 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   if (!rc) rc = dalloc(&a->zref, (size_t)n);
   if (!rc) rc = dalloc(&a->ms, (size_t)n);
   if (!rc) rc = dalloc(&a->acc, (size_t)2 * n);
-  if (!rc) rc = a->work.ensure(n, g0.H, g0.W, false);
   if (rc) return rc;
   CHIP(hipMemcpy(a->bstd, bed_std, spx * n * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemcpy(a->zref, zref, n * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemset(a->ms, 0, n * sizeof(ClMass)));
   CHIP(hipMemset(a->acc, 0, 2 * n * sizeof(double)));
+  std::vector<ClDom> hd(n);
+  for (int s2 = 0; s2 < n; s2++)
+    hd[s2] = ClDom{fip_internal::grid(fip_internal::ens_member(ens, s2)), a->bstd + (size_t)s2 * spx, zref[s2], 0};
+  if ((rc = dalloc(&a->dom, (size_t)n))) return rc;
+  CHIP(hipMemcpy(a->dom, hd.data(), n * sizeof(ClDom), hipMemcpyHostToDevice));
   Attachment* ap = a.get();
   cl->att.push_back(std::move(a));
   fip_internal::ens_set_post_hook(ens, [cl, ap, n](hipStream_t st, int par) -> int {
     const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ap->ens, 0));
-    CHIP(cl_features_ens_launch(fip_internal::ens_scn(ap->ens), ap->bstd, ap->zref, par, ap->work.x0, g0.W, g0.H,
-                                n, st));
-    int rc = forward(cl, ap->work, nullptr, 0, st);
-    if (rc) return rc;
-    const fip_ens_scn* scn = fip_internal::ens_scn(ap->ens);
-    CHIP(cl_correct_ens_launch(scn, par, ap->work.o, kBound, ap->sat, ap->ms, g0.W, g0.H, n, st));
-    CHIP(cl_mass_sum_ens_launch(scn, par, ap->acc, g0.W, g0.H, n, st));
-    CHIP(cl_mass_check_ens_launch(scn, n, par, ap->acc, ap->ms, ap->limit, st));
+    CHIP(cl_fused_launch(ap->dom, n, g0.W, g0.H, par, cl->wf, cl->wl1, cl->w, kBound, ap->sat, ap->ms, ap->acc,
+                         nullptr, st));
+    CHIP(cl_mass_check_ens_launch(fip_internal::ens_scn(ap->ens), n, par, ap->acc, ap->ms, ap->limit, st));
     return FIP_OK;
   });
   return FIP_OK;
@@ -589,6 +605,35 @@ int fip_closure_backward(fip_closure* cl, const float* feat, int N, int H, int W
   if ((rc = read_loss(ws, cl->stream, &r, &m, &p))) return rc;
   if (loss) *loss = r / (2.0 * (double)npix);
   CHIP(hipMemcpy(grad, cl->g, sizeof(float) * CL_LAYERS * CL_LAYER_PARAMS, hipMemcpyDeviceToHost));
+  return FIP_OK;
+}
+
+int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed_std, float zref, float* out) {
+  if (!cl || !coarse || !bed_std || !out) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(cl->dev));
+  CHIP(hipDeviceSynchronize());
+  const fip_grid& g = fip_internal::grid(coarse);
+  const size_t spx = (size_t)g.W * g.H;
+  float *bstd = nullptr, *o = nullptr;
+  ClDom* dom = nullptr;
+  ClMass* ms = nullptr;
+  double* acc = nullptr;
+  CHIP(hipMalloc(&bstd, spx * sizeof(float)));
+  CHIP(hipMalloc(&o, spx * 2 * sizeof(float)));
+  CHIP(hipMalloc(&dom, sizeof(ClDom)));
+  CHIP(hipMalloc(&ms, sizeof(ClMass)));
+  CHIP(hipMalloc(&acc, 2 * sizeof(double)));
+  CHIP(hipMemset(ms, 0, sizeof(ClMass)));
+  CHIP(hipMemcpy(bstd, bed_std, spx * sizeof(float), hipMemcpyHostToDevice));
+  ClDom hd{g, bstd, zref, 0};
+  CHIP(hipMemcpy(dom, &hd, sizeof hd, hipMemcpyHostToDevice));
+  // par such that par ^ 1 is the current side and the halt flag read is that of no step
+  const int par = fip_internal::cur_side(coarse) ^ 1;
+  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, kBound, 1e30f, ms, acc, o, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, o, spx * 2 * sizeof(float), hipMemcpyDeviceToHost);
+  hipFree(bstd); hipFree(o); hipFree(dom); hipFree(ms); hipFree(acc);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("fused outputs: ") + hipGetErrorString(e));
   return FIP_OK;
 }
 

@@ -47,6 +47,11 @@ extern "C" __global__ __launch_bounds__(256) void fip_ens_post(const fip_ens_scn
   __shared__ float s_zv[FIP_MAX_BC], s_swd[FIP_MAX_BC];
   __shared__ int s_need[FIP_MAX_BC];
   __shared__ float s_terms[256];
+  __shared__ fip_ens_bc s_bc[FIP_MAX_BC];
+  // the scenario's BC descriptors, staged by the whole work-group: thread 0's serial part
+  // then reads them from LDS instead of one dependent global load per BC
+  for (int i = tid; i < nbc; i += blockDim.x) s_bc[i] = bcs[bc0 + i];
+  __syncthreads();
   if (sidx == 0 && tid == 0 && mode == 1) {
     ec->list_count[par] = 0u;
     ec->work_next[par] = 0u;
@@ -78,7 +83,7 @@ extern "C" __global__ __launch_bounds__(256) void fip_ens_post(const fip_ens_scn
     if (!skip) {
       const float tf = (float)dur;
       for (int b = 0; b < nbc; b++) {
-        const fip_ens_bc B = bcs[bc0 + b];
+        const fip_ens_bc& B = s_bc[b];
         const float zv = ens_bc_z(B, series, tf);
         s_zv[b] = zv;
         s_need[b] = B.type == FIP_BCT_Q &&
@@ -92,7 +97,7 @@ extern "C" __global__ __launch_bounds__(256) void fip_ens_post(const fip_ens_scn
   // sumWaterDepths where z changed: terms in parallel, summed by thread 0 in order
   for (int b = 0; b < nbc; b++) {
     if (!s_need[b]) continue;
-    const fip_ens_bc B = bcs[bc0 + b];
+    const fip_ens_bc& B = s_bc[b];
     const float zv = s_zv[b];
     float ah_sum = 0.f;
     for (int base = B.from; base < B.to; base += 256) {
@@ -114,7 +119,7 @@ extern "C" __global__ __launch_bounds__(256) void fip_ens_post(const fip_ens_scn
   if (tid != 0) return;
   const float tf = (float)dur;
   for (int b = 0; b < nbc; b++) {
-    const fip_ens_bc B = bcs[bc0 + b];
+    const fip_ens_bc& B = s_bc[b];
     float qv;
     if (B.type == FIP_BCT_Q) {
       float swd = c->swd_val[b];
@@ -200,7 +205,7 @@ extern "C" __global__ __launch_bounds__(256) void fip_ens_apply(
   }
   const fip_ens_chunk ch = chunks[blockIdx.x];
   if ((int)threadIdx.x >= ch.n) return;
-  const fip_grid g = scn[ch.scn].g;
+  const fip_grid& g = scn[ch.scn].g;
   const fip_ctl* c = g.ctl;
   if (mode == 1 && c->halt_req[par ^ 1]) return;
   const int cur = mode == 1 ? (par ^ 1) : par;
@@ -214,7 +219,7 @@ extern "C" __global__ __launch_bounds__(256) void fip_ens_apply(
 // flood_plains_kernel (src/kernels.cu:254-262) of every scenario whose iteration is a
 // multiple of 100 (src/main.cpp:125-128); idempotent. blockIdx.y = scenario.
 extern "C" __global__ void fip_ens_floodplain(const fip_ens_scn* scn) {
-  const fip_grid g = scn[blockIdx.y].g;
+  const fip_grid& g = scn[blockIdx.y].g;
   const int it = g.ctl->iteration;
   if ((it % 100) != 0) return;
   const float* h = g.h[it & 1];

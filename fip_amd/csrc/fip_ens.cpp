@@ -67,7 +67,7 @@ struct fip_ens {
   int total_tiles = 0;
   int cus = 256;
   int wpb = 4;
-  int bpc = 8;  // persistent work-groups per CU (0 = one wavefront per list entry)
+  int bpc = 0;  // persistent work-groups per CU (0 = one wavefront per list entry)
   int stencil_blocks = 0;
   int sched_blocks = 0;
   long long iter = 0;
@@ -324,6 +324,8 @@ int fip_ens_destroy(fip_ens* e) {
   if (!e) return FIP_OK;
   hipSetDevice(e->dev);
   if (e->stream) hipStreamSynchronize(e->stream);
+  fip_internal::closure_forget(nullptr, e);
+  e->post_hook = nullptr;
   destroy_graphs(e);
   free_tables(e);
   if (e->d_list) hipFree(e->d_list);
@@ -383,7 +385,7 @@ int fip_ens_set_launch(fip_ens* e, int waves_per_block, int blocks_per_cu) {
   if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu > 64)
     return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu at most 64");
   e->wpb = waves_per_block == 0 ? 4 : waves_per_block;
-  e->bpc = blocks_per_cu < 0 ? 8 : blocks_per_cu;
+  e->bpc = blocks_per_cu < 0 ? 0 : blocks_per_cu;
   destroy_graphs(e);
   return FIP_OK;
 }

@@ -43,4 +43,8 @@ int ens_count(const fip_ens* e);
 fip_sim* ens_member(const fip_ens* e, int s);
 hipStream_t ens_stream(const fip_ens* e);
 
+// Called by fip_destroy / fip_ens_destroy: closures drop their attachments to the object
+// being destroyed (without touching it).
+void closure_forget(const fip_sim* s, const fip_ens* e);
+
 }  // namespace fip_internal

@@ -73,7 +73,7 @@ struct fip_sim {
   bool fast = false;
   int cus = 256;           // compute units of the device
   int wpb = 4;             // wavefronts per stencil work-group
-  int bpc = 8;             // masked mode: persistent work-groups per CU (0 = one wave per entry)
+  int bpc = 0;             // masked mode: persistent work-groups per CU (0 = one wave per entry)
   int stencil_blocks = 0;
   int sched_blocks = 0;
   // boundary kernel
@@ -638,6 +638,8 @@ int fip_destroy(fip_sim* s) {
   if (!s) return FIP_OK;
   hipSetDevice(s->dev);
   if (s->stream) hipStreamSynchronize(s->stream);
+  fip_internal::closure_forget(s, nullptr);
+  s->post_hook = nullptr;
   destroy_graphs(s);
   for (auto& p : s->base_alloc)
     if (p) hipFree(p);
@@ -1025,7 +1027,7 @@ int fip_set_launch(fip_sim* s, int waves_per_block, int blocks_per_cu) {
   if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu > 64)
     return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu at most 64");
   s->wpb = waves_per_block == 0 ? 4 : waves_per_block;
-  s->bpc = blocks_per_cu < 0 ? 8 : blocks_per_cu;
+  s->bpc = blocks_per_cu < 0 ? 0 : blocks_per_cu;
   s->stencil_blocks = std::max(1, s->cus * std::max(1, s->bpc));
   destroy_graphs(s);
   return FIP_OK;

@@ -162,7 +162,7 @@ int fip_get_masks(fip_sim* sim, uint64_t* dst, int n);
 int fip_get_cfl(fip_sim* sim, float* cfl_min);
 /* Launch shape of the stencil: wavefronts per work-group (1..16; 0 = 4) and, in masked
  * mode, persistent work-groups per CU (1..64), or 0 for a grid of one wavefront per
- * active-tile entry (< 0 = default 8). Results do not depend on it. */
+ * active-tile entry (< 0 = default: 0). Results do not depend on it. */
 int fip_set_launch(fip_sim* sim, int waves_per_block, int blocks_per_cu);
 
 typedef struct fip_tune_result {

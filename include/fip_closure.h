@@ -85,6 +85,10 @@ int fip_closure_forward(fip_closure* cl, const float* feat, int N, int H, int W,
 int fip_closure_backward(fip_closure* cl, const float* feat, int N, int H, int W, const float* qp,
                          const float* qt, const unsigned char* mask, float bound, float* grad,
                          double* loss);
+/* The raw outputs [H][W][2] of the fused inference kernel on the coarse simulation's
+ * current state (no correction applied): checks it against fip_closure_forward. */
+int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed_std, float zref,
+                              float* out);
 /* D = A B of one bf16 matrix-core instruction (A 16x32, B 32x16 row-major bf16 bits,
  * D 16x16 fp32): checks the operand layout the convolutions assume. */
 int fip_mfma_probe(int device, const unsigned short* A, const unsigned short* B, float* D);

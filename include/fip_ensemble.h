@@ -36,7 +36,7 @@ int fip_ens_get_time(fip_ens* e, int s, double* duration, float* dt, int* iterat
 int fip_ens_download(fip_ens* e, int s, int field, int side, float* dst);
 /* Launch shape of the ensemble stencil: wavefronts per work-group (1..16; 0 = 4) and
  * persistent work-groups per CU (1..64), or 0 for a grid of one wavefront per active-tile
- * entry (< 0 = default 8). Results do not depend on it. */
+ * entry (< 0 = default: 0). Results do not depend on it. */
 int fip_ens_set_launch(fip_ens* e, int waves_per_block, int blocks_per_cu);
 /* Autotuner of the ensemble (fip_tune for ensembles): times the ensemble stencil of
  * these scenarios (bcs[s] holds nbcs[s] BCs of scenario s) for every tile-row count and

@@ -128,3 +128,13 @@ def test_metrics():
     a[1, 1] = 0.5
     assert co.wet_iou(a, b) == pytest.approx(0.5)
     assert co.interior_face_mask(8, 8)[:, :, 0].sum() == 4 * 5
+
+
+def test_product_metrics_equal_oracle():
+    from fip_amd import metrics
+    rng = np.random.default_rng(4)
+    a = np.maximum(0, rng.normal(0, 0.3, (20, 30)))
+    b = np.maximum(0, rng.normal(0, 0.3, (20, 30)))
+    assert metrics.depth_rmse(a, b) == co.depth_rmse(a, b)
+    assert metrics.max_depth_error(a, b) == co.max_depth_error(a, b)
+    assert metrics.wet_iou(a, b) == co.wet_iou(a, b)

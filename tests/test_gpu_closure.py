@@ -181,3 +181,47 @@ def test_inference_hook_mass_and_fallback():
         x.close()
     cl.close()
     big.close()
+
+
+def test_fused_step_matches_layerwise_forward():
+    """One coarse step with the closure attached equals the plain step plus
+    tanh(o) 0.1 |q| on interior faces, o from the layer-by-layer network on the plain
+    step's state (tolerance: the fused kernel sums layer 0 in another order, bf16)."""
+    from fip_amd.sim import Closure, Sim
+    fine, coarse, bstd, zref = fine_coarse_case(384, 256, 1.0, 4, scenario=6, wet=0.5)
+    coarse.h = coarse.h + np.float32(0.0)
+    rng = np.random.default_rng(8)
+    cl = Closure(96, 64, 0, seed=2)
+    cl.params = rand_params(rng, 0.3)
+    a = Sim(coarse, mode="dense")
+    a.set_run(1e9)
+    a.simulate(30)
+    b = Sim(coarse, mode="dense")
+    b.set_run(1e9)
+    b.simulate(29)
+    cl.attach(b, bstd, zref, sat=50.0)
+    # the attached step starts from the same state as a's last step
+    b.simulate(1)
+    ha, qxa, qya = a.field("h"), a.field("qx"), a.field("qy")
+    feat = co.features(ha, qxa, qya, coarse.bed, bstd, coarse.kst, zref)[None]
+    o = cl.forward(feat)[0]
+    of = cl.fused_outputs(a, bstd, zref)
+    err = np.abs(of - o)
+    print("fused vs layerwise outputs: max", err.max(), "at", np.unravel_index(err.argmax(), err.shape),
+          "scale", np.abs(o).max(), flush=True)
+    assert err.max() <= 2e-2 * max(1.0, np.abs(o).max())
+    H, W = ha.shape
+    m = co.interior_face_mask(H, W)
+    ok = np.abs(o) <= 50.0  # faces where the network saturates keep the plain step
+    assert ok.any() and (~ok).any()
+    exp_x = qxa + np.where((m[..., 0] == 1) & ok[..., 0], np.tanh(o[..., 0]) * 0.1 * np.abs(qxa), 0)
+    exp_y = qya + np.where((m[..., 1] == 1) & ok[..., 1], np.tanh(o[..., 1]) * 0.1 * np.abs(qya), 0)
+    got_x, got_y = b.field("qx"), b.field("qy")
+    cx = np.abs(exp_x - qxa).max()
+    assert cx > 0
+    assert np.abs(got_x - exp_x).max() <= 0.05 * cx + 1e-7
+    assert np.abs(got_y - exp_y).max() <= 0.05 * np.abs(exp_y - qya).max() + 1e-7
+    assert np.array_equal(b.field("h"), ha)
+    for x in (a, b):
+        x.close()
+    cl.close()
