@@ -11,7 +11,9 @@ Prints one JSON line: value = W*H*steps / wall seconds (Gcell-updates/s) in dens
 the roofline of the stencil kernel (algorithmic 32 B per cell with kSt_var, timed with HIP
 events on the simulation's stream), the masked-mode effective rate, the exact-variant
 rate, the batched ensemble of 64 scenarios of uneven sizes (launch shape from the
-ensemble autotuner) and the CPU oracle on a bounded sample.
+ensemble autotuner), the learned subgrid closure (training on solver-produced pairs,
+then learned coarse vs plain fine ensembles on held-out scenarios: speed and depth RMSE,
+max-depth error, wet-extent IoU) and the CPU oracle on a bounded sample.
 """
 from __future__ import annotations
 
@@ -157,6 +159,11 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure leg")
+    ap.add_argument("--closure-train", type=int, default=16, help="fine training scenarios")
+    ap.add_argument("--closure-test", type=int, default=8, help="held-out scenarios")
+    ap.add_argument("--closure-duration", type=float, default=1800.0)
+    ap.add_argument("--closure-steps", type=int, default=4000, help="Adam steps")
     ap.add_argument("--calibrate", action="store_true",
                     help="run 1 GiB device copies (counter calibration) and exit")
     args = ap.parse_args()
@@ -218,6 +225,20 @@ def main():
                          "tiles_total": tiles[1]}
     if not args.no_extra and not args.no_ensemble:
         out["ensemble"] = run_ensemble(args.variant, args.ensemble_steps, 50)
+    if not args.no_extra and not args.no_closure:
+        from fip_amd import closure_run
+        res = closure_run.experiment(n_train=args.closure_train, n_test=args.closure_test,
+                                     duration=args.closure_duration, steps=args.closure_steps,
+                                     log=lambda m: print(m, file=sys.stderr, flush=True))
+        ev = res.pop("eval")
+        rmse_ok = ev["learned_depth_rmse_m"] <= 0.02
+        out["closure"] = {
+            "workload": (f"{args.closure_train} training + {args.closure_test} held-out scenarios, fine "
+                         f"1024^2 dx=1 m (ridged terrain, z inflow), coarse 256^2 dx=4 m, "
+                         f"{args.closure_duration:.0f} s simulated, fixed dt 0.05/0.2 s"),
+            "speedup_learned_vs_fine": ev["speedup_learned_vs_fine"],
+            "speedup_at_rmse_le_2cm": ev["speedup_learned_vs_fine"] if rmse_ok else None,
+            "training": res, **{k: v for k, v in ev.items() if k not in ("scenarios", "duration_s")}}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         rate, secs = cpu_baseline(case, args.cpu_steps, threads)

@@ -23,24 +23,24 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-// float -> bf16, round to nearest even (NaN kept quiet)
-__device__ __forceinline__ u16 f2bf(float f) {
-  uint32_t u = __builtin_bit_cast(uint32_t, f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (u16)(u >> 16);
-}
+// float -> bf16, round to nearest even (the native conversion of gfx950)
+__device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
 __device__ __forceinline__ float bf2f(u16 b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
+
+// tanh(u) = 1 - 2 / (1 + e^(2u)) with the hardware exp and reciprocal (saturates to +-1)
+__device__ __forceinline__ float fast_tanh(float u) {
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * u));
+}
 
 // GELU, tanh form: 0.5 x (1 + tanh(s (x + 0.044715 x^3))), s = sqrt(2/pi)
 __device__ __forceinline__ float gelu(float x) {
   const float s = 0.7978845608028654f;
-  return 0.5f * x * (1.f + tanhf(s * (x + 0.044715f * x * x * x)));
+  return 0.5f * x * (1.f + fast_tanh(s * (x + 0.044715f * x * x * x)));
 }
 __device__ __forceinline__ float dgelu(float x) {
   const float s = 0.7978845608028654f;
   const float u = s * (x + 0.044715f * x * x * x);
-  const float t = tanhf(u);
+  const float t = fast_tanh(u);
   return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * s * (1.f + 3.f * 0.044715f * x * x);
 }
 
