@@ -8,7 +8,8 @@ post-step boundary/control kernel (and the flood-plain kernel every 100 steps), 
 the device-resident loop in hipGraph chunks. Inputs are resident in HBM before timing.
 
 Prints one JSON line: value = W*H*steps / wall seconds (Gcell-updates/s) in dense mode,
-the roofline of the stencil kernel (algorithmic 32 B per cell with kSt_var, timed with HIP
+the roofline of the stencil kernel (algorithmic bytes per cell: h, qx, qy, bed read and
+h, qx, qy written, 28 B, plus kSt: 4 B as a float or 1 B as a class index; timed with HIP
 events on the simulation's stream), the masked-mode effective rate, the exact-variant
 rate, the batched ensemble of 64 scenarios of uneven sizes (launch shape from the
 ensemble autotuner), the learned subgrid closure (training on solver-produced pairs,
@@ -29,7 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = {False: 28, True: 32}  # uniform kSt / kSt_var
+
+def bytes_per_cell(kvar: bool, kst_classes: int) -> int:
+    """Algorithmic bytes per cell update: 16 read + 12 written, plus kSt_var as a float
+    (4 B) or as a one-byte class index."""
+    return 28 + (0 if not kvar else 1 if kst_classes else 4)
 
 
 def wet_state(case, frac=0.30):
@@ -62,7 +67,7 @@ def run_rate(case, mode, variant, rows, steps, warmup, timed_steps):
     s.set_timing(False)
     ms = (s1["stencil_ms"] - s0["stencil_ms"]) / max(1, s1["timed_launches"] - s0["timed_launches"])
     s.close()
-    return (t1 - t0), ms, tiles
+    return (t1 - t0), ms, tiles, s1
 
 
 def run_ensemble(variant, steps, warmup, nscn=64):
@@ -179,10 +184,10 @@ def main():
     case = wet_state(config1())
     W, H = case.W, case.H
     kvar = case.kst_var is not None
-    bpc = BYTES_PER_CELL[kvar]
 
-    wall, ms, _ = run_rate(case, "dense", args.variant, args.rows, args.steps, args.warmup,
-                           args.timed_steps)
+    wall, ms, _, st = run_rate(case, "dense", args.variant, args.rows, args.steps, args.warmup,
+                               args.timed_steps)
+    bpc = bytes_per_cell(kvar, st["kst_classes"])
     value = W * H * args.steps / wall / 1e9
     achieved = bpc * W * H / (ms * 1e-3) / 1e9
     traffic = measured_traffic(args.variant, args.rows)
@@ -202,7 +207,9 @@ def main():
         "config": {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q BC, "
                                "right open, 30% wet, dense mode",
                    "W": W, "H": H, "mode": "dense", "variant": args.variant,
-                   "tile": f"60x{args.rows}", "parallelism": "single GPU"},
+                   "tile": f"60 columns x {st['dense_span']}-row spans, 8 wavefronts in step",
+                   "kst": f"{st['kst_classes']} classes, 1 B per cell" if st["kst_classes"] else "f32",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "stencil_dense", "kernel_ms": round(ms, 4),
@@ -211,13 +218,13 @@ def main():
     }
     if not args.no_extra:
         other = "exact" if args.variant == "fast" else "fast"
-        w2, ms2, _ = run_rate(case, "dense", other, args.rows, max(100, args.steps // 4),
+        w2, ms2, _, _ = run_rate(case, "dense", other, args.rows, max(100, args.steps // 4),
                               args.warmup, args.timed_steps)
         n2 = max(100, args.steps // 4)
         out[f"dense_{other}"] = {"gcells": round(W * H * n2 / w2 / 1e9, 3),
                                  "kernel_ms": round(ms2, 4),
                                  "roofline_frac": round(bpc * W * H / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        w3, ms3, tiles = run_rate(case, "masked", args.variant, args.rows, max(100, args.steps // 4),
+        w3, ms3, tiles, _ = run_rate(case, "masked", args.variant, args.rows, max(100, args.steps // 4),
                                   args.warmup, args.timed_steps)
         n3 = max(100, args.steps // 4)
         out["masked"] = {"gcells_effective": round(W * H * n3 / w3 / 1e9, 3),

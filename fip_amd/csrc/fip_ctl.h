@@ -8,6 +8,9 @@
 #define FIP_MAX_SAMPLE_POINTS 64
 #define FIP_SAMPLE_LOG_ROWS 256
 #define FIP_MAX_SAVE 256
+#ifndef FIP_KST_CLASSES
+#define FIP_KST_CLASSES 64
+#endif
 
 // Stop requests raised by the post-step kernel (bits).
 #define FIP_HALT_FINISHED 1  // duration >= target: the loop of src/main.cpp:243 ends
@@ -60,6 +63,11 @@ typedef struct fip_grid {
   float* hmax;
   const float* bed;
   const float* kst;       // NULL when kSt is uniform
+  // kSt by class: when kSt_var takes at most FIP_KST_CLASSES distinct values, kidx holds
+  // the class of every cell (same element layout as the float fields, one byte per
+  // element) and ktab the class values; NULL otherwise
+  const unsigned char* kidx;
+  const float* ktab;
   unsigned long long* mask;   // one 64-bit row mask per tile
   unsigned* list[2];          // active tiles: (tile << 12) | (lo << 6) | hi
   const float* series;        // BC time-series tables

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/fip.h"
@@ -17,9 +18,11 @@
 #include "fip_internal.h"
 
 hipError_t fip_launch_stencil_exact(const fip_grid& g, int par, int masked, int blocks, int wpb,
-                                    hipStream_t st);
+                                    int span, hipStream_t st);
 hipError_t fip_launch_stencil_fast(const fip_grid& g, int par, int masked, int blocks, int wpb,
-                                   hipStream_t st);
+                                   int span, hipStream_t st);
+hipError_t fip_stencil_dense_waves_per_cu_exact(int kvar, int wpb, int* waves);
+hipError_t fip_stencil_dense_waves_per_cu_fast(int kvar, int wpb, int* waves);
 hipError_t fip_launch_misc(int which, const fip_grid& g, int arg, int blocks, hipStream_t st);
 
 namespace {
@@ -65,6 +68,8 @@ struct fip_sim {
   hipStream_t stream = nullptr;
   fip_grid g{};
   float* base_alloc[9] = {};
+  unsigned char* kidx_alloc = nullptr;  // kSt classes (see fip_grid::kidx)
+  float* ktab_alloc = nullptr;
   size_t field_elems = 0;  // elements per allocation (with margins)
   long interior_off = 0;   // offset of element (0,0)
   int ntiles = 0;
@@ -72,10 +77,12 @@ struct fip_sim {
   bool masked = false;
   bool fast = false;
   int cus = 256;           // compute units of the device
-  int wpb = 4;             // wavefronts per stencil work-group
+  int wpb = 4;             // wavefronts per stencil work-group (see default_wpb)
   int bpc = 0;             // masked mode: persistent work-groups per CU (0 = one wave per entry)
   int stencil_blocks = 0;
   int sched_blocks = 0;
+  int dense_span = 0;      // dense mode: rows per wavefront
+  int dense_blocks = 0;    // dense mode: work-groups per step
   // boundary kernel
   hipModule_t module = nullptr;
   hipFunction_t bc_post_fn = nullptr;
@@ -100,13 +107,43 @@ struct fip_sim {
 
 namespace {
 
+// Dense mode: one wavefront per span of rows of a tile column, the wpb wavefronts of a
+// work-group on adjacent tile columns of one span row (the last group of a span row may
+// hold idle wavefronts past the grid). Spans are sized so that the whole grid is resident
+// at once (every SIMD holds its full share of equally long wavefronts: no tail of a few
+// wavefronts running alone). Interior spans march span + 2 rows (two warm-up rows);
+// span + 2 is kept a multiple of the 6-row register ring.
+int plan_dense(fip_sim* s) {
+  const int kv = s->g.kidx ? 2 : s->g.kst ? 1 : 0;
+  int wpc = 0;
+  hipError_t e = s->fast ? fip_stencil_dense_waves_per_cu_fast(kv, s->wpb, &wpc)
+                         : fip_stencil_dense_waves_per_cu_exact(kv, s->wpb, &wpc);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("occupancy query: ") + hipGetErrorString(e));
+  if (wpc < 1) return fail(FIP_E_HIP, "the dense stencil kernel cannot be resident");
+  const long slots = (long)s->cus * wpc;
+  const int tx = s->g.tiles_x, H = s->g.H;
+  const long per_col = std::max<long>(1, slots / ((long)(tx + s->wpb - 1) / s->wpb * s->wpb));
+  int span = (int)((H + per_col - 1) / per_col);
+  span = std::max(span, 16);
+  while ((span + 2) % 6) span++;
+  const int nspans = (H + span - 1) / span;
+  s->dense_span = span;
+  s->dense_blocks = nspans * ((tx + s->wpb - 1) / s->wpb);
+  return FIP_OK;
+}
+
+// Wavefronts per work-group by default: 8 for the dense fast variant (its work-groups
+// march in step), 4 otherwise.
+int default_wpb(const fip_sim* s) { return !s->masked && s->fast ? 8 : 4; }
+
 int launch_stencil(fip_sim* s, int par) {
   const int kind = s->masked ? (s->bpc > 0 ? 2 : 1) : 0;
-  const int blocks = kind == 2 ? s->stencil_blocks : (s->ntiles + s->wpb - 1) / s->wpb;
+  const int blocks = kind == 2 ? s->stencil_blocks
+                     : kind == 1 ? (s->ntiles + s->wpb - 1) / s->wpb : s->dense_blocks;
   hipError_t e;
   if (s->timing) HIPTRY(hipEventRecord(s->ev0, s->stream));
-  e = s->fast ? fip_launch_stencil_fast(s->g, par, kind, blocks, s->wpb, s->stream)
-              : fip_launch_stencil_exact(s->g, par, kind, blocks, s->wpb, s->stream);
+  e = s->fast ? fip_launch_stencil_fast(s->g, par, kind, blocks, s->wpb, s->dense_span, s->stream)
+              : fip_launch_stencil_exact(s->g, par, kind, blocks, s->wpb, s->dense_span, s->stream);
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
   if (s->timing) {
     HIPTRY(hipEventRecord(s->ev1, s->stream));
@@ -533,6 +570,7 @@ int fip_create(const fip_desc* d, fip_sim** out) {
   s->dev = d->device;
   s->masked = d->mode == FIP_MODE_MASKED;
   s->fast = d->variant == FIP_FAST;
+  s->wpb = default_wpb(s);
   auto cleanup = [&](int rc) {
     fip_destroy(s);
     return rc;
@@ -596,6 +634,39 @@ int fip_create(const fip_desc* d, fip_sim** out) {
   rc |= up((float*)g.bed, d->bed);
   if (d->kst_var) rc |= up((float*)g.kst, d->kst_var);
   if (rc) return cleanup(FIP_E_HIP);
+  if (d->kst_var) {
+    // kSt by class: the stencil then reads one byte per cell instead of four and looks
+    // the value up, which gives the same float
+    std::unordered_map<uint32_t, int> cls;
+    std::vector<float> tab;
+    std::vector<unsigned char> idx((size_t)W * H);
+    bool ok = true;
+    for (size_t i = 0; ok && i < (size_t)W * H; i++) {
+      uint32_t bits;
+      std::memcpy(&bits, &d->kst_var[i], 4);
+      auto it = cls.find(bits);
+      if (it == cls.end()) {
+        if ((int)tab.size() == FIP_KST_CLASSES) { ok = false; break; }
+        it = cls.emplace(bits, (int)tab.size()).first;
+        tab.push_back(d->kst_var[i]);
+      }
+      idx[i] = (unsigned char)it->second;
+    }
+    if (ok) {
+      while ((int)tab.size() < FIP_KST_CLASSES) tab.push_back(tab[0]);
+      if (hipMalloc(&s->kidx_alloc, s->field_elems) != hipSuccess ||
+          hipMemset(s->kidx_alloc, 0, s->field_elems) != hipSuccess ||
+          hipMalloc(&s->ktab_alloc, sizeof(float) * FIP_KST_CLASSES) != hipSuccess)
+        return cleanup(fail(FIP_E_HIP, "kSt class allocation failed"));
+      g.kidx = s->kidx_alloc + s->interior_off;
+      g.ktab = s->ktab_alloc;
+      s->stats.kst_classes = (int)cls.size();
+      if (hipMemcpy2D(s->kidx_alloc + s->interior_off, (size_t)P, idx.data(), (size_t)W, (size_t)W, H,
+                      hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(s->ktab_alloc, tab.data(), sizeof(float) * FIP_KST_CLASSES, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(FIP_E_HIP, "kSt class upload failed"));
+    }
+  }
   // masks: every tile wet in every row (src/simdata.cpp:77-79)
   std::vector<unsigned long long> masks(s->ntiles);
   for (int t = 0; t < s->ntiles; t++) {
@@ -626,6 +697,7 @@ int fip_create(const fip_desc* d, fip_sim** out) {
   s->cus = cus;
   s->stencil_blocks = std::max(1, cus * std::max(1, s->bpc));
   s->sched_blocks = std::max(1, std::min((s->ntiles + 255) / 256, cus));
+  if (int rc2 = plan_dense(s)) return cleanup(rc2);
   if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipEventCreate failed"));
   s->stats.tiles_total = s->ntiles;
@@ -641,6 +713,8 @@ int fip_destroy(fip_sim* s) {
   fip_internal::closure_forget(s, nullptr);
   s->post_hook = nullptr;
   destroy_graphs(s);
+  if (s->kidx_alloc) hipFree(s->kidx_alloc);
+  if (s->ktab_alloc) hipFree(s->ktab_alloc);
   for (auto& p : s->base_alloc)
     if (p) hipFree(p);
   if (s->g.mask) hipFree(s->g.mask);
@@ -1026,23 +1100,26 @@ int fip_set_launch(fip_sim* s, int waves_per_block, int blocks_per_cu) {
   if (!s) return fail(FIP_E_INVALID, "null argument");
   if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu > 64)
     return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu at most 64");
-  s->wpb = waves_per_block == 0 ? 4 : waves_per_block;
+  s->wpb = waves_per_block != 0 ? waves_per_block : default_wpb(s);
   s->bpc = blocks_per_cu < 0 ? 0 : blocks_per_cu;
   s->stencil_blocks = std::max(1, s->cus * std::max(1, s->bpc));
   destroy_graphs(s);
-  return FIP_OK;
+  HIPTRY(hipSetDevice(s->dev));
+  return plan_dense(s);
 }
 
 int fip_tune(const fip_desc* d, const fip_bc* bcs, int nbc, int steps, fip_tune_result* out) {
   if (!d || !out || (nbc > 0 && !bcs)) return fail(FIP_E_INVALID, "null argument");
   if (steps < 1) return fail(FIP_E_INVALID, "steps must be at least 1");
   static const int kRows[] = {16, 32, 64};
-  static const int kWaves[] = {1, 2, 4, 8};
+  static const int kWaves[] = {1, 2, 4, 8, 16};
   static const int kBlocksPerCu[] = {0, 4, 8, 16};
   const bool masked = d->mode == FIP_MODE_MASKED;
   fip_tune_result best{};
   best.stencil_ms = 1e30;
   for (int R : kRows) {
+    // dense mode marches spans sized at launch: the tile rows do not reach its kernel
+    if (!masked && R != (d->tile_rows == 0 ? 64 : d->tile_rows)) continue;
     fip_desc dd = *d;
     dd.tile_rows = R;
     fip_sim* s = nullptr;
@@ -1089,6 +1166,7 @@ int fip_set_timing(fip_sim* s, int enable) {
 int fip_get_stats(fip_sim* s, fip_stats* out) {
   if (!s || !out) return fail(FIP_E_INVALID, "null argument");
   *out = s->stats;
+  out->dense_span = s->masked ? 0 : s->dense_span;
   return FIP_OK;
 }
 

@@ -53,7 +53,8 @@ class FipBC(C.Structure):
 class FipStats(C.Structure):
     _fields_ = [("steps", C.c_longlong), ("tiles_total", C.c_longlong),
                 ("tiles_computed", C.c_longlong), ("stencil_launches", C.c_longlong),
-                ("stencil_ms", C.c_double), ("timed_launches", C.c_longlong)]
+                ("stencil_ms", C.c_double), ("timed_launches", C.c_longlong),
+                ("kst_classes", C.c_int), ("dense_span", C.c_int)]
 
 
 class FipTune(C.Structure):

@@ -282,7 +282,10 @@ def small_case(W: int, H: int, scenario: int = 0, wet: bool = True, kst_var: boo
         level = np.float32(np.percentile(bed32, 40))
         h = np.maximum(np.float32(0), level - bed32).astype(np.float32)
     kv = None
-    if kst_var:
+    if kst_var == "many":  # about 1000 distinct values: kSt is read as floats
+        jj, ii = np.mgrid[0:H, 0:W]
+        kv = (20.0 + ((ii * 7919 + jj * 104729) % 1000) * 0.025).astype(np.float32)
+    elif kst_var:  # five values: kSt is read as a one-byte class per cell
         kv = np.empty((H, W), dtype=np.float32)
         for j in range(H):
             for i in range(W):

@@ -108,6 +108,9 @@ typedef struct fip_stats {
   long long stencil_launches;
   double stencil_ms;          /* summed stencil kernel time when timing is enabled */
   long long timed_launches;
+  int kst_classes;            /* kSt_var read as a one-byte class per cell: the number of
+                                 classes (at most 64); 0 when kSt is read as floats */
+  int dense_span;             /* dense mode: rows per wavefront */
 } fip_stats;
 
 const char* fip_last_error(void);
@@ -156,13 +159,16 @@ int fip_get_time(fip_sim* sim, double* duration, float* dt, int* iteration);
 int fip_set_dt(fip_sim* sim, float dt);
 int fip_download(fip_sim* sim, int field, int side, float* dst);
 int fip_upload(fip_sim* sim, int field, int side, const float* src);
-/* Tile geometry and the 64-bit row masks (bit k = row tile_rows*ty + k ended wet). */
+/* Tile geometry and the 64-bit row masks (bit k = row tile_rows*ty + k ended wet). Masks
+ * are kept in masked mode; dense mode (ENABLE_MASK 0) leaves them as created. */
 int fip_tiles(fip_sim* sim, int* tiles_x, int* tiles_y, int* tile_cols, int* tile_rows);
 int fip_get_masks(fip_sim* sim, uint64_t* dst, int n);
 int fip_get_cfl(fip_sim* sim, float* cfl_min);
-/* Launch shape of the stencil: wavefronts per work-group (1..16; 0 = 4) and, in masked
- * mode, persistent work-groups per CU (1..64), or 0 for a grid of one wavefront per
- * active-tile entry (< 0 = default: 0). Results do not depend on it. */
+/* Launch shape of the stencil: wavefronts per work-group (1..16; 0 = the default: 8 for
+ * the dense fast variant, 4 otherwise) and, in masked mode, persistent work-groups per CU
+ * (1..64), or 0 for a grid of one wavefront per active-tile entry (< 0 = default: 0).
+ * In dense mode the wavefronts of a work-group take adjacent tile columns (the fast
+ * variant marches them in step). Results do not depend on it. */
 int fip_set_launch(fip_sim* sim, int waves_per_block, int blocks_per_cu);
 
 typedef struct fip_tune_result {
@@ -174,10 +180,11 @@ typedef struct fip_tune_result {
 } fip_tune_result;
 
 /* Autotuner (no counterpart in the reference, whose tile shape is fixed at 28 x 50 by
- * src/kernels.cu:441-455): times the stencil of this configuration for every tile-row
- * count and launch shape over `steps` timed steps (after as many warm-up steps) from the
- * initial state, and returns the fastest. In dense mode results do not depend on the
- * choice; in masked mode the tile rows decide which tiles are skipped. */
+ * src/kernels.cu:441-455): times the stencil of this configuration for every launch
+ * shape (and, in masked mode, every tile-row count) over `steps` timed steps (after as
+ * many warm-up steps) from the initial state, and returns the fastest. In dense mode
+ * results do not depend on the choice and tile_rows is returned as given; in masked mode
+ * the tile rows decide which tiles are skipped. */
 int fip_tune(const fip_desc* desc, const fip_bc* bcs, int nbc, int steps, fip_tune_result* out);
 int fip_set_timing(fip_sim* sim, int enable);
 int fip_get_stats(fip_sim* sim, fip_stats* out);

@@ -51,6 +51,7 @@ CASES = {
     "wet_97x83": dict(W=97, H=83, wet=True),
     "dry_z_inflow_130x70": dict(W=130, H=70, wet=False),
     "kvar_nodata_121x90": dict(W=121, H=90, wet=True, kst_var=True, no_data=True),
+    "kvar_many_110x75": dict(W=110, H=75, wet=True, kst_var="many"),
     "q_series_100x64": dict(W=100, H=64, wet=False, bc_kinds=("q", "open", "close", "close")),
     "all_open_91x77": dict(W=91, H=77, wet=True, bc_kinds=("open", "open", "open", "open")),
 }
@@ -71,7 +72,8 @@ def test_dense_exact_fine_grained(name):
             assert np.float32(g.cfl()) == np.float32(o.cfl_min())
 
 
-@pytest.mark.parametrize("name", ["wet_97x83", "q_series_100x64", "kvar_nodata_121x90"])
+@pytest.mark.parametrize("name", ["wet_97x83", "q_series_100x64", "kvar_nodata_121x90",
+                                  "kvar_many_110x75"])
 def test_dense_exact_device_resident(name):
     case = small_case(**CASES[name])
     case.duration = 25.0
@@ -91,7 +93,8 @@ def test_dense_exact_device_resident(name):
 
 
 @pytest.mark.parametrize("rows", [16, 32, 64])
-@pytest.mark.parametrize("name", ["wet_97x83", "dry_z_inflow_130x70", "kvar_nodata_121x90"])
+@pytest.mark.parametrize("name", ["wet_97x83", "dry_z_inflow_130x70", "kvar_nodata_121x90",
+                                  "kvar_many_110x75"])
 def test_masked_exact_vs_tiled_oracle(name, rows):
     case = small_case(**CASES[name])
     g = sim(case, mode="masked", tile_rows=rows)
@@ -156,8 +159,11 @@ def test_bc_params_and_host_launch():
     assert_time_equal(g, o)
 
 
-def test_fast_variant_within_tolerance():
-    case = small_case(W=150, H=120, wet=True)
+@pytest.mark.parametrize("kst_var", [False, True, "many"])
+def test_fast_variant_within_tolerance(kst_var):
+    """One step of the fast variant from the exact run's state (uniform kSt, kSt read by
+    class, kSt read as floats): fields within 1e-5, CFL minimum within 1e-5 relative."""
+    case = small_case(W=150, H=120, wet=True, kst_var=kst_var)
     e = sim(case)
     for _ in range(20):
         e.step_reference_order()
@@ -172,6 +178,9 @@ def test_fast_variant_within_tolerance():
     for k in ("h", "qx", "qy"):
         d = np.abs(e.field(k).astype(np.float64) - f.field(k).astype(np.float64))
         assert d.max() <= 1e-5, (k, d.max())
+    assert abs(f.cfl() - e.cfl()) <= 1e-5 * e.cfl(), (f.cfl(), e.cfl())
+    classes = f.stats()["kst_classes"]
+    assert classes == (5 if kst_var is True else 0), classes
 
 
 def test_sampling_and_dumps():

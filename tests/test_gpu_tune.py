@@ -51,11 +51,11 @@ def test_tune_single_domain():
     from fip_amd.sim import tune
     case = small_case(300, 200, wet=True)
     r = tune(case, mode="dense", variant="fast", steps=5)
-    assert r["tried"] == 12 and r["blocks_per_cu"] == -1
-    assert r["tile_rows"] in (16, 32, 64) and r["waves_per_block"] in (1, 2, 4, 8)
+    assert r["tried"] == 5 and r["blocks_per_cu"] == -1
+    assert r["tile_rows"] in (16, 32, 64) and r["waves_per_block"] in (1, 2, 4, 8, 16)
     assert 0 < r["stencil_ms"] < 100
     r = tune(case, mode="masked", variant="fast", steps=5)
-    assert r["tried"] == 48 and r["blocks_per_cu"] in (0, 4, 8, 16)
+    assert r["tried"] == 60 and r["blocks_per_cu"] in (0, 4, 8, 16)
 
 
 def test_tune_ensemble():
