@@ -104,6 +104,20 @@ def build_cli(verbose: bool = False) -> str | None:
     return CLI
 
 
+PROBE = os.path.join(PKG, "bin", "stream_probe")
+
+
+def build_probe(verbose: bool = False) -> str:
+    """The memory-ceiling probe of the dense stencil's traffic (csrc/stream_probe.hip)."""
+    os.makedirs(os.path.dirname(PROBE), exist_ok=True)
+    tmp = PROBE + ".tmp"
+    _run_all([[HIPCC] + COMMON + ["-o", tmp, os.path.join(CSRC, "stream_probe.hip")]])
+    os.replace(tmp, PROBE)
+    if verbose:
+        print("built", PROBE)
+    return PROBE
+
+
 def build_oracle(verbose: bool = False) -> str:
     d = os.path.join(ROOT, "oracle")
     subprocess.run(["make", "-s", "-C", d, "-B"], check=True)
@@ -118,6 +132,7 @@ def build_oracle(verbose: bool = False) -> str:
 def build_all(verbose: bool = False) -> None:
     build_lib(verbose)
     build_cli(verbose)
+    build_probe(verbose)
     build_oracle(verbose)
 
 

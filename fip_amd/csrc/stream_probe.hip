@@ -1,0 +1,149 @@
+// Memory ceiling of the dense stencil's traffic on this device, without its arithmetic.
+//
+// The stencil step moves 29 B per cell on the 5667 x 6500 workload: h, qx, qy and bed are
+// read as floats, the kSt class as one byte, and h, qx, qy are written. This program moves
+// the same bytes in three ways and prints the bandwidth of each:
+//   march     the stencil's own access pattern: one wavefront per 60-column tile column
+//             (64 lanes: 4 halo columns), marching a span of rows (two warm-up rows), one
+//             4-byte (or 1-byte) access per lane and field per row; stores skip the halo
+//             lanes and warm-up rows;
+//   march/8   the same with the 8 wavefronts of a work-group on adjacent tile columns,
+//             stepping through the rows in step (a barrier every second row), as the fast
+//             dense kernel does;
+//   flat      the same byte counts as one flat stream: 16-byte accesses per lane
+//             (4 bytes per lane for the class field), grid-stride;
+//   copy      a plain 16-byte-per-lane copy of one field (1 read + 1 write).
+// Every access stays inside its allocation: rows are below H, columns are clamped to
+// [0, W - 1], flat indices are below W * H / 4.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+namespace {
+
+constexpr int kW = 5667, kH = 6500, kP = 5696;  // pitch: a multiple of 64 floats
+constexpr int kIn = 4, kOut = 3;
+
+struct Fields {
+  const float* in[kIn];
+  const unsigned char* cls;
+  float* out[kOut];
+};
+
+template <bool STEP>
+__global__ __launch_bounds__(512) void march(Fields f, int span, int tiles_x, int groups) {
+  const int wpb = blockDim.x >> 6, lane = threadIdx.x & 63;
+  const int sr = blockIdx.x / groups;
+  const int tx = (blockIdx.x % groups) * wpb + (threadIdx.x >> 6);
+  int c = tx * 60 + lane - 2;
+  const bool own = lane >= 2 && lane < 62 && c >= 0 && c < kW;
+  c = c < 0 ? 0 : (c >= kW ? kW - 1 : c);
+  const int gfirst = sr * span;
+  const int r0 = gfirst >= 2 ? gfirst - 2 : 0;
+  const int r1 = gfirst + span < kH ? gfirst + span : kH;
+  float acc = 0.f;
+  for (int r = r0; r < r1; ++r) {
+    if (STEP && ((r - r0) & 1) == 0) __builtin_amdgcn_s_barrier();
+    const size_t o = (size_t)r * kP + c;
+    float s = f.cls[o];
+#pragma unroll
+    for (int k = 0; k < kIn; k++) s += f.in[k][o];
+    acc = acc * 0.5f + s;
+    if (own && r >= gfirst) {
+#pragma unroll
+      for (int k = 0; k < kOut; k++) f.out[k][o] = acc + (float)k;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void flat(Fields f, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const uchar4 cl = reinterpret_cast<const uchar4*>(f.cls)[i];
+    float4 s = make_float4(cl.x, cl.y, cl.z, cl.w);
+#pragma unroll
+    for (int k = 0; k < kIn; k++) {
+      const float4 v = reinterpret_cast<const float4*>(f.in[k])[i];
+      s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    }
+#pragma unroll
+    for (int k = 0; k < kOut; k++) reinterpret_cast<float4*>(f.out[k])[i] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void copy(const float4* a, float4* b, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) b[i] = a[i];
+}
+
+#define CK(x)                                                                             \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      std::fprintf(stderr, "%s: %s\n", #x, hipGetErrorString(e_));                        \
+      std::exit(1);                                                                       \
+    }                                                                                     \
+  } while (0)
+
+template <class F>
+float time_ms(F launch, int reps) {
+  hipEvent_t a, b;
+  CK(hipEventCreate(&a));
+  CK(hipEventCreate(&b));
+  for (int i = 0; i < 3; i++) launch();
+  CK(hipEventRecord(a));
+  for (int i = 0; i < reps; i++) launch();
+  CK(hipEventRecord(b));
+  CK(hipEventSynchronize(b));
+  float ms = 0.f;
+  CK(hipEventElapsedTime(&ms, a, b));
+  CK(hipEventDestroy(a));
+  CK(hipEventDestroy(b));
+  return ms / reps;
+}
+
+}  // namespace
+
+int main() {
+  const size_t n = (size_t)kP * kH;
+  Fields f{};
+  for (int k = 0; k < kIn; k++) {
+    float* p = nullptr;
+    CK(hipMalloc(&p, n * sizeof(float)));
+    CK(hipMemset(p, 0, n * sizeof(float)));
+    f.in[k] = p;
+  }
+  unsigned char* cl = nullptr;
+  CK(hipMalloc(&cl, n));
+  CK(hipMemset(cl, 1, n));
+  f.cls = cl;
+  for (int k = 0; k < kOut; k++) CK(hipMalloc(&f.out[k], n * sizeof(float)));
+  const double bytes = 29.0 * kW * kH;
+  const int tiles_x = (kW + 59) / 60, reps = 20;
+  std::printf("| pattern | shape | ms | GB/s (29 B x %d x %d) |\n|---|---|---|---|\n", kW, kH);
+  for (int span : {32, 106}) {
+    const int nsp = (kH + span - 1) / span;
+    for (int wpb : {4, 8}) {
+      const int groups = (tiles_x + wpb - 1) / wpb;
+      const dim3 grid(nsp * groups), block(64 * wpb);
+      float ms = time_ms([&] { hipLaunchKernelGGL(march<false>, grid, block, 0, 0, f, span, tiles_x, groups); }, reps);
+      std::printf("| march | %d-row spans, %d waves/group | %.4f | %.0f |\n", span, wpb, ms, bytes / ms / 1e6);
+      ms = time_ms([&] { hipLaunchKernelGGL(march<true>, grid, block, 0, 0, f, span, tiles_x, groups); }, reps);
+      std::printf("| march/%d in step | %d-row spans | %.4f | %.0f |\n", wpb, span, ms, bytes / ms / 1e6);
+    }
+  }
+  const size_t n4 = (size_t)kW * kH / 4;
+  for (int blocks : {2048, 8192}) {
+    const float ms = time_ms([&] { hipLaunchKernelGGL(flat, dim3(blocks), dim3(256), 0, 0, f, n4); }, reps);
+    std::printf("| flat | %d blocks of 256 | %.4f | %.0f |\n", blocks, ms, 29.0 * 4 * n4 / ms / 1e6);
+  }
+  const float ms = time_ms([&] {
+    hipLaunchKernelGGL(copy, dim3(8192), dim3(256), 0, 0, reinterpret_cast<const float4*>(f.in[0]),
+                       reinterpret_cast<float4*>(f.out[0]), n4);
+  }, reps);
+  std::printf("| copy | 1 read + 1 write, 8192 blocks | %.4f | %.0f (8 B per element) |\n", ms,
+              8.0 * 4 * n4 / ms / 1e6);
+  CK(hipDeviceSynchronize());
+  return 0;
+}
