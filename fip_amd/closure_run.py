@@ -1,13 +1,21 @@
 """Training data, training and evaluation of the learned subgrid closure, driven through
 the C ABI (include/fip_closure.h); every step runs on the GPU.
 
-Fine scenarios (dx = 1 m) follow the config-0 recipe with 1 m-scale ridges: a plane
-sloping 0.002, Gaussian hills with sigma ~ U(3, 20) cells, a 20-cell channel 3 m deep
-along y = H/2, `nridges` straight ridges 2 cells wide and U(0.3, 1.5) m high, uniform
-kSt ~ U(20, 45); left type-z inflow over the channel rows (z = bed + 2 m), right open, top
-and bottom closed; dry start. The coarse twin is the 4x restriction (bed = block mean,
-z BC levels = the mean of the four fine levels). Both run with a fixed dt (0.05 s fine,
-0.2 s coarse: one coarse step = four fine steps, the pairing the closure is trained on).
+Fine scenarios (dx = 1 m) follow the config-0 terrain recipe with 1 m-scale ridges: a
+plane sloping 0.002, Gaussian hills with sigma ~ U(3, 20) cells, a 24-cell channel 3 m deep
+along y = H/2 (aligned with the 4x blocks), `nridges` straight ridges 2 cells wide and
+U(0.3, 1.5) m high, uniform kSt ~ U(20, 45); dry start; right open, top and bottom closed.
+The inflow is a left type-q boundary over the channel rows plus one dry bank row: Q ramps
+from 0 (60 s before t = 0) to Q0 ~ U(20, 60) m^3/s at t = 0 and stays there, with the
+water level of the discharge distribution at the channel bed + 2 m. A discharge boundary
+puts the same volume into the fine domain and its coarse twin (the bank row, dry on both
+grids, is the `to` cell that src/boundary_condition.cpp:10 leaves out of the sum); with a
+fixed-level (type z) inflow the two grids draw different volumes through the boundary,
+which no interior flux correction can change, and the judged depth errors would be set
+by the boundary rather than by the terrain. The coarse twin is the 4x restriction (bed =
+block mean, the same discharge boundary on the coarse rows). Both run with a fixed dt
+(0.05 s fine, 0.2 s coarse: one coarse step = four fine steps, the pairing the closure is
+trained on).
 
 Training pairs: every `sample_every` seconds of a fine run, its state is restricted onto
 the coarse grid, the coarse solver takes one plain step, the fine solver four steps, and
@@ -58,14 +66,16 @@ def fine_case(scenario: int, W: int = 1024, H: int = 1024, nridges: int = 200, d
     bed = np.broadcast_to(0.002 * dx * (W - 1 - x), (H, W)).copy()
     nh = max(4, int(round(64 * W * H / (512.0 * 512.0))))
     add_hills(bed, rng, nh, (0.0, 5.0), (3.0, 20.0))
-    c = H // 2
-    bed[c - 10:c + 10, :] -= 3.0
+    c = (H // 2) // F * F
+    bed[c - 12:c + 12, :] -= 3.0
     bed += _ridges(H, W, rng, nridges)
     kst = float(np.float32(rng.uniform(20.0, 45.0)))
+    q0 = rng.uniform(20.0, 60.0)
     bed32 = bed.astype(np.float32)
-    bcs = [BCSpec("z", "left", row, row, z=float(bed32[row, 0] + np.float32(2.0)))
-           for row in range(c - 20, c + 20)]
-    bcs += [BCSpec("open", "right"), BCSpec("close", "top"), BCSpec("close", "bottom")]
+    zc = float(np.float32(bed32[c - 12:c + 12, 0].min()) + np.float32(2.0))
+    bcs = [BCSpec("q", "left", c - 12, c + 12, z_series=([0.0, duration], [zc, zc]),
+                  q_series=([-60.0, 0.0, duration], [0.0, q0, q0])),
+           BCSpec("open", "right"), BCSpec("close", "top"), BCSpec("close", "bottom")]
     return Case(name=f"fine_{scenario}", W=W, H=H, dx=dx, dt=DT_FINE, duration=duration, bed=bed32,
                 kst=kst, variable_dt=False, bcs=bcs)
 
@@ -76,12 +86,9 @@ def coarse_of(fine: Case) -> tuple[Case, np.ndarray, float]:
     b = fine.bed.astype(np.float64).reshape(H, F, W, F)
     cb = (b.sum(axis=(1, 3)) / (F * F)).astype(np.float32)
     bstd = np.sqrt(((b - b.mean(axis=(1, 3), keepdims=True)) ** 2).mean(axis=(1, 3))).astype(np.float32)
-    zl = {}
-    for bc in fine.bcs:
-        if bc.type == "z":
-            zl.setdefault(bc.frm // F, []).append(bc.z)
-    bcs = [BCSpec("z", "left", r, r, z=float(np.float32(np.mean(v)))) for r, v in sorted(zl.items())]
-    bcs += [BCSpec("open", "right"), BCSpec("close", "top"), BCSpec("close", "bottom")]
+    bcs = [BCSpec(bc.type, bc.side, bc.frm // F if bc.frm is not None else None,
+                  bc.to // F if bc.to is not None else None, z=bc.z, q=bc.q,
+                  z_series=bc.z_series, q_series=bc.q_series) for bc in fine.bcs]
     coarse = Case(name=fine.name + "_c", W=W, H=H, dx=fine.dx * F, dt=fine.dt * F, duration=fine.duration,
                   bed=cb, kst=fine.kst, variable_dt=False, bcs=bcs)
     return coarse, bstd, float(cb.mean())
@@ -93,7 +100,9 @@ def block_mean(a: np.ndarray) -> np.ndarray:
 
 
 def generate_samples(cl, scenarios, duration, sample_every=60.0, variant="fast", W=1024, log=print):
-    """Adds (duration / sample_every) samples per scenario to the closure's store."""
+    """Adds (duration / sample_every) samples per scenario to the closure's store: the input
+    of a sample is the restricted state of the fine run, the target the restricted fine
+    discharges one coarse step (four fine steps) later."""
     from .sim import Sim, restrict
     per = int(round(sample_every / DT_FINE)) - F
     t0 = time.perf_counter()
@@ -115,7 +124,7 @@ def generate_samples(cl, scenarios, duration, sample_every=60.0, variant="fast",
     log(f"samples: {cl.num_samples()} in {time.perf_counter() - t0:.1f} s")
 
 
-def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print):
+def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.65):
     """Plain fine vs plain coarse vs learned coarse on held-out scenarios, as ensembles.
     Returns metrics and timings."""
     from . import metrics as co
@@ -131,7 +140,7 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print):
     def run(cases, closure):
         e = Ensemble(cases, mode="dense", variant=variant, tile_rows=32)
         if closure is not None:
-            closure.attach_ensemble(e, bstd, zref)
+            closure.attach_ensemble(e, bstd, zref, sat=sat)
         e.set_run([duration] * len(cases))
         e.simulate(200)  # graphs built, warm
         t0 = time.perf_counter()
@@ -175,19 +184,21 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print):
 
 def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1500, batch=32, patch=128,
                lr=1e-3, W=1024, log=print):
+    """Generate samples, train, evaluate. The held-out samples come first in the store, the
+    training samples after them."""
     from .sim import Closure
     Wc = W // F
     nsamp = n_train * int(duration // sample_every) + n_test * 4
     cl = Closure(Wc, Wc, max_samples=nsamp, seed=11)
-    generate_samples(cl, range(n_train), duration, sample_every, W=W, log=log)
-    ntr = cl.num_samples()
     generate_samples(cl, range(1000, 1000 + n_test), 4 * sample_every, sample_every, W=W, log=log)
-    plain0, learned0 = cl.eval(ntr, cl.num_samples() - ntr)
+    ntest = cl.num_samples()
+    generate_samples(cl, range(n_train), duration, sample_every, W=W, log=log)
+    plain0, learned0 = cl.eval(0, ntest)
     t0 = time.perf_counter()
-    losses = cl.train(steps, batch=batch, patch=patch, lr=lr, first=0, n=ntr)
+    losses = cl.train(steps, batch=batch, patch=patch, lr=lr, first=ntest)
     twall = time.perf_counter() - t0
-    plain1, learned1 = cl.eval(ntr, cl.num_samples() - ntr)
-    res = {"train_samples": ntr, "train_steps": steps, "train_wall_s": round(twall, 2),
+    plain1, learned1 = cl.eval(0, ntest)
+    res = {"train_samples": cl.num_samples() - ntest, "train_steps": steps, "train_wall_s": round(twall, 2),
            "loss_first10": float(np.mean(losses[:10])), "loss_last50": float(np.mean(losses[-50:])),
            "heldout_flux_mse_plain": plain1, "heldout_flux_mse_learned": learned1,
            "heldout_flux_mse_untrained": learned0}

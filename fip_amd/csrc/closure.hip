@@ -26,16 +26,25 @@ namespace {
 // float -> bf16, round to nearest even (the native conversion of gfx950)
 __device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
 __device__ __forceinline__ float bf2f(u16 b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
+// two floats -> bf16 pair (a in the low half), one conversion instruction
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b) {
+  typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, (bf2){(__bf16)a, (__bf16)b});
+}
 
 // tanh(u) = 1 - 2 / (1 + e^(2u)) with the hardware exp and reciprocal (saturates to +-1)
 __device__ __forceinline__ float fast_tanh(float u) {
   return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * u));
 }
 
-// GELU, tanh form: 0.5 x (1 + tanh(s (x + 0.044715 x^3))), s = sqrt(2/pi)
+// GELU, tanh form: 0.5 x (1 + tanh(s (x + 0.044715 x^3))), s = sqrt(2/pi), evaluated as
+// x / (1 + 2^(k x (1 + 0.044715 x^2))) with k = -2 s log2(e) (0.5 (1 + tanh u) is the
+// logistic function of 2u): seven instructions, two of them transcendental. Saturates to
+// x and to -0 for large |x|.
 __device__ __forceinline__ float gelu(float x) {
-  const float s = 0.7978845608028654f;
-  return 0.5f * x * (1.f + fast_tanh(s * (x + 0.044715f * x * x * x)));
+  constexpr float k = -2.f * 0.7978845608028654f * 1.4426950408889634f;
+  const float t = x * __builtin_fmaf(k * 0.044715f, x * x, k);
+  return x * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(t));
 }
 __device__ __forceinline__ float dgelu(float x) {
   const float s = 0.7978845608028654f;
@@ -601,10 +610,16 @@ __device__ __forceinline__ void fused_layer(const u16* in, u16* out, const u16* 
   const int gx0 = ox - (ROUT - FT) / 2, gy0 = oy - (ROUT - FT) / 2;
   f32x4 acc[TPW][MT];
   int base[TPW];
+  f32x4 bias4[MT];  // the accumulators start at the bias of their output channels
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int co0 = m * 16 + 4 * grp;
+    bias4[m] = (f32x4){bias[co0], bias[co0 + 1], bias[co0 + 2], bias[co0 + 3]};
+  }
 #pragma unroll
   for (int i = 0; i < TPW; i++) {
 #pragma unroll
-    for (int m = 0; m < MT; m++) acc[i][m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MT; m++) acc[i][m] = bias4[m];
     const int p = (wave + FNW * i) * 16 + col;
     const int pp = p < NPIX ? p : NPIX - 1;
     base[i] = (pp / ROUT + off) * RIN + pp % ROUT + off;   // input pixel of tap (0, 0)
@@ -638,23 +653,20 @@ __device__ __forceinline__ void fused_layer(const u16* in, u16* out, const u16* 
     const bool inside = gx0 + xx >= 0 && gx0 + xx < W && gy0 + yy >= 0 && gy0 + yy < H;
     if (o2) {  // last layer: outputs 0 and 1, fp32, held by lane group 0
       if (grp == 0) {
-        o2[p * 2] = acc[i][0][0] + bias[0];
-        o2[p * 2 + 1] = acc[i][0][1] + bias[1];
+        o2[p * 2] = acc[i][0][0];
+        o2[p * 2 + 1] = acc[i][0][1];
       }
       continue;
     }
 #pragma unroll
     for (int m = 0; m < MT; m++) {
       const int co0 = m * 16 + 4 * grp;
-      uint32_t lo, hi;
-      {
-        const u16 z0 = f2bf(acc[i][m][0] + bias[co0]), z1 = f2bf(acc[i][m][1] + bias[co0 + 1]);
-        const u16 z2 = f2bf(acc[i][m][2] + bias[co0 + 2]), z3 = f2bf(acc[i][m][3] + bias[co0 + 3]);
-        const u16 a0 = inside ? f2bf(gelu(bf2f(z0))) : (u16)0, a1 = inside ? f2bf(gelu(bf2f(z1))) : (u16)0;
-        const u16 a2 = inside ? f2bf(gelu(bf2f(z2))) : (u16)0, a3 = inside ? f2bf(gelu(bf2f(z3))) : (u16)0;
-        lo = (uint32_t)a0 | ((uint32_t)a1 << 16);
-        hi = (uint32_t)a2 | ((uint32_t)a3 << 16);
-      }
+      // GELU of the fp32 pre-activation (the layer-by-layer form rounds it to bf16 first,
+      // which it stores for the backward pass; inference has no use for that rounding),
+      // packed to bf16 pairs; cells outside the domain are set to 0 by a select
+      const uint32_t keep = 0u - (uint32_t)inside;
+      const uint32_t lo = pack_bf2(gelu(acc[i][m][0]), gelu(acc[i][m][1])) & keep;
+      const uint32_t hi = pack_bf2(gelu(acc[i][m][2]), gelu(acc[i][m][3])) & keep;
       *(uint2*)(out + (yy * ROUT + xx) * FPS + co0) = make_uint2(lo, hi);
     }
   }
