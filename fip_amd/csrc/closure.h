@@ -51,7 +51,7 @@ struct ClDom {
 
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
                            const float* w, float bound, float sat, ClMass* ms, double* acc, float* o_out,
-                           hipStream_t st);
+                           hipStream_t st, int form = 1);
 hipError_t cl_pack_l1_launch(const float* w, u16* wl1, hipStream_t st);
 hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
                           u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,

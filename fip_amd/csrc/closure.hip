@@ -757,6 +757,338 @@ extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, 
 }
 
 // ---------------------------------------------------------------------------------------
+// Persistent form of the fused inference step: the same per-tile arithmetic (the same
+// fragments summed in the same order, so the outputs are bit-identical to cl_fused), with
+// the latencies of the tile form taken off the critical path:
+//   * one work-group per CU takes a contiguous run of tiles; the packed weights and biases
+//     of all four layers are staged in LDS once per launch, not re-read from global memory
+//     by every wavefront of every tile;
+//   * the input cells of tile t+1 (bed, h, qx, qy, subgrid std, kSt) are loaded into
+//     registers while tile t runs layer 0, and written to the feature buffer after layer 1;
+//   * the correction takes the pre-correction discharges and depths of its 16 x 16 cells
+//     from LDS (double-buffered), so it issues stores only;
+//   * the mass terms are summed per thread over the work-group's tiles of one domain and
+//     reduced once per domain. Each box-boundary discharge is added by the thread of the
+//     cell that owns the face, from the pre-correction values, so no thread reads a face
+//     that another one corrects.
+// Four barriers per tile. LDS: 52 KB of weights + 80 KB of activations (one work-group
+// per CU, 8 wavefronts).
+#define FP_PIX (FR0 * FR0)                    // 576 feature pixels per tile
+#define FP_PER ((FP_PIX + 511) / 512)         // 2 per thread
+#define FP_W1 CL_PACKED_L1                    // u16 offsets of the layers in the LDS weights
+#define FP_W2 (FP_W1 + CL_PACKED)
+#define FP_W3 (FP_W2 + CL_PACKED)
+#define FP_WN (FP_W3 + CL_PACKED / 2)         // layer 3 uses output block 0 only
+
+// One layer of the persistent fused step: the arithmetic of fused_layer (the same
+// fragments summed in the same order, so the same bits). The wave holds the A fragments
+// of every K chunk of the layer in registers and walks its pixel tiles one after the
+// other; the epilogue (bias-initialised accumulators -> GELU -> bf16 -> LDS) of tile i-1
+// is issued between the matrix instructions of tile i, so the vector ALU work overlaps
+// the matrix cores. The code is branch-free: the tile count K is a template argument
+// (uniform per wave), and lanes past the region's last pixel store to a spare pixel slot
+// (index NPIX) of the output buffer, or to a per-lane spare slot of o2.
+template <int RIN, int ROUT, int MT, bool L1, int K>
+__device__ __forceinline__ void fp_layer_k(const u16* in, u16* out, const u16* wp, const float* bias, int ox,
+                                           int oy, int W, int H, float* o2, int wave) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  constexpr int NPIX = ROUT * ROUT;
+  constexpr int off = (RIN - ROUT) / 2;
+  constexpr int NC = L1 ? 3 : CL_TAPS;
+  const int gx0 = ox - (ROUT - FT) / 2, gy0 = oy - (ROUT - FT) / 2;
+  f32x4 b4[MT];
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int co0 = m * 16 + 4 * grp;
+    b4[m] = (f32x4){bias[co0], bias[co0 + 1], bias[co0 + 2], bias[co0 + 3]};
+  }
+  bf16x8 a[NC][MT];
+#pragma unroll
+  for (int c = 0; c < NC; c++)
+#pragma unroll
+    for (int m = 0; m < MT; m++) a[c][m] = ld8(wp + ((m * NC + c) * 64 + lane) * 8);
+  const auto pix = [&](int i) { return (wave + FNW * i) * 16 + col; };
+  const auto ldB = [&](int c, int i) -> bf16x8 {
+    const int p = pix(i);
+    const int pp = p < NPIX ? p : NPIX - 1;
+    const int base = (pp / ROUT + off) * RIN + pp % ROUT + off;
+    if constexpr (L1) {
+      const int tap = 4 * c + grp;
+      const int sh = tap < CL_TAPS ? (tap / 3 - 1) * RIN + (tap % 3 - 1) : 0;
+      const bf16x8 v = ld8(in + (base + sh) * 8);
+      return tap < CL_TAPS ? v : zero8();
+    } else {
+      const int sh = (c / 3 - 1) * RIN + (c % 3 - 1);
+      return ld8(in + (base + sh) * FPS + 8 * grp);
+    }
+  };
+  const auto epilogue = [&](int i, const f32x4 (&acc)[MT]) {
+    const int p = pix(i);
+    const bool valid = p < NPIX;
+    if (o2) {  // last layer: outputs 0 and 1, fp32, held by lane group 0
+      const int dst = (grp == 0 && valid) ? p * 2 : 1024 + 2 * lane;
+      o2[dst] = acc[0][0];
+      o2[dst + 1] = acc[0][1];
+      return;
+    }
+    const int yy = p / ROUT, xx = p % ROUT;
+    const bool inside = gx0 + xx >= 0 && gx0 + xx < W && gy0 + yy >= 0 && gy0 + yy < H;
+    const uint32_t keep = 0u - (uint32_t)inside;
+    const int px = valid ? p : NPIX;
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      const int co0 = m * 16 + 4 * grp;
+      const uint32_t lo = pack_bf2(gelu(acc[m][0]), gelu(acc[m][1])) & keep;
+      const uint32_t hi = pack_bf2(gelu(acc[m][2]), gelu(acc[m][3])) & keep;
+      *(uint2*)(out + px * FPS + co0) = make_uint2(lo, hi);
+    }
+  };
+  f32x4 prev[MT];
+#pragma unroll
+  for (int i = 0; i <= K; i++) {
+    f32x4 acc[MT];
+    if (i < K) {
+#pragma unroll
+      for (int m = 0; m < MT; m++) acc[m] = b4[m];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const bf16x8 b = ldB(c, i);
+#pragma unroll
+        for (int m = 0; m < MT; m++) acc[m] = mfma(a[c][m], b, acc[m]);
+      }
+    }
+    if (i > 0) epilogue(i - 1, prev);
+    if (i < K) {
+#pragma unroll
+      for (int m = 0; m < MT; m++) prev[m] = acc[m];
+    }
+  }
+}
+
+// tiles are dealt round-robin, so a wave has TPW or TPW - 1 of them
+template <int RIN, int ROUT, int MT, bool L1>
+__device__ __forceinline__ void fp_layer(const u16* in, u16* out, const u16* wp, const float* bias, int ox, int oy,
+                                         int W, int H, float* o2) {
+  constexpr int NT = (ROUT * ROUT + 15) / 16, TPW = (NT + FNW - 1) / FNW;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nt = (NT - wave + FNW - 1) / FNW;
+  if (nt == TPW)
+    fp_layer_k<RIN, ROUT, MT, L1, TPW>(in, out, wp, bias, ox, oy, W, H, o2, wave);
+  else if constexpr (TPW > 1)
+    fp_layer_k<RIN, ROUT, MT, L1, TPW - 1>(in, out, wp, bias, ox, oy, W, H, o2, wave);
+}
+
+struct FpTile {   // one tile of the persistent loop (wave-uniform)
+  int d, ox, oy;
+};
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ FpTile fp_tile(int t, int tx, int ty) {
+  const int per = tx * ty, r = t % per;
+  return FpTile{uni(t / per), uni((r % tx) * FT), uni((r / tx) * FT)};
+}
+
+// first tile in [t, end) whose domain did not halt this step (end if none)
+__device__ __forceinline__ int fp_next(const ClDom* __restrict__ dom, int t, int end, int tx, int ty, int par) {
+  const int per = tx * ty;
+  t = uni(t);
+  while (t < end) {
+    const int d = uni(t / per);
+    if (!dom[d].g.ctl->halt_req[par ^ 1]) break;
+    t = uni((d + 1) * per);
+  }
+  return t < end ? t : end;
+}
+
+// What a tile needs of its domain, read once (scalar loads) one tile ahead of its use.
+struct FpDom {
+  const float *h, *bed, *kst, *bstd;
+  float *qx, *qy;
+  float kst_u, zref;
+  int W, H, P, fallback;
+};
+
+__device__ __forceinline__ FpDom fp_dom(const ClDom* __restrict__ dom, int d, int cur, const ClMass* ms) {
+  const ClDom& D = dom[d];
+  return FpDom{D.g.h[cur], D.g.bed, D.g.kst, D.bstd, D.g.qx[cur], D.g.qy[cur], D.g.kst_uniform, D.zref,
+               D.g.W, D.g.H, D.g.P, uni(ms[d].fallback)};
+}
+
+struct FpCells {  // the raw inputs of this thread's feature pixels of one tile
+  float b[FP_PER], h[FP_PER], qx[FP_PER], qy[FP_PER], bs[FP_PER], k[FP_PER];
+};
+
+__device__ __forceinline__ void fp_load(const FpDom& D, FpTile T, FpCells& c) {
+#pragma unroll
+  for (int j = 0; j < FP_PER; j++) {
+    const int e = threadIdx.x + 512 * j;
+    const int gx = T.ox - 4 + e % FR0, gy = T.oy - 4 + e / FR0;
+    c.b[j] = c.h[j] = c.qx[j] = c.qy[j] = c.bs[j] = c.k[j] = 0.f;
+    if (e < FP_PIX && gx >= 0 && gx < D.W && gy >= 0 && gy < D.H) {
+      const long o = (long)gy * D.P + gx;
+      c.b[j] = D.bed[o];
+      c.h[j] = D.h[o];
+      c.qx[j] = D.qx[o];
+      c.qy[j] = D.qy[o];
+      c.bs[j] = D.bstd[(long)gy * D.W + gx];
+      c.k[j] = D.kst ? D.kst[o] : D.kst_u;
+    }
+  }
+}
+
+// features (the formulas of feat_pixel, as in cl_fused) into f0; the 16 x 16 core cells'
+// qx, qy and h into core
+__device__ __forceinline__ void fp_store(const FpDom& D, FpTile T, const FpCells& c, u16* f0, float* core) {
+#pragma unroll
+  for (int j = 0; j < FP_PER; j++) {
+    const int e = threadIdx.x + 512 * j;
+    if (e >= FP_PIX) continue;
+    const int lx = e % FR0 - 4, ly = e / FR0 - 4;
+    const int gx = T.ox + lx, gy = T.oy + ly;
+    uint4 pk = make_uint4(0, 0, 0, 0);
+    if (gx >= 0 && gx < D.W && gy >= 0 && gy < D.H) {
+      const float b = c.b[j];
+      const float f0v = (c.h[j] + b - D.zref) * 0.25f;
+      const float f3v = (b - D.zref) * 0.25f;
+      const float f4v = c.bs[j] * 2.0f;
+      const float f5v = c.k[j] * (1.0f / 30.0f);
+      pk.x = (uint32_t)f2bf(f0v) | ((uint32_t)f2bf(c.qx[j]) << 16);
+      pk.y = (uint32_t)f2bf(c.qy[j]) | ((uint32_t)f2bf(f3v) << 16);
+      pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
+    }
+    *(uint4*)(f0 + e * 8) = pk;
+    if (lx >= 0 && lx < FT && ly >= 0 && ly < FT) {
+      float* q = core + (ly * FT + lx) * 3;
+      q[0] = c.qx[j];
+      q[1] = c.qy[j];
+      q[2] = c.h[j];
+    }
+  }
+}
+
+extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_eu(2, 2))) void cl_fused_p(
+    const ClDom* __restrict__ dom, int par, const u16* wf, const u16* wl1, const float* w, float bound, float sat,
+    ClMass* ms, double* acc, float* o_out, int tx, int ty, int ntiles, int per_wg) {
+  __shared__ __attribute__((aligned(16))) u16 sw[FP_WN];
+  __shared__ float sb[CL_LAYERS * CL_C];
+  __shared__ __attribute__((aligned(16))) u16 f0[FR0 * FR0 * 8];
+  __shared__ __attribute__((aligned(16))) u16 b1[(FR1 * FR1 + 1) * FPS];  // + a spare pixel slot
+  __shared__ __attribute__((aligned(16))) u16 b2[(FR2 * FR2 + 1) * FPS];
+  __shared__ float core[2][FT * FT * 3];
+  __shared__ double red[FNW * 2];
+  const int t0 = uni(blockIdx.x * per_wg);
+  const int end = uni(t0 + per_wg < ntiles ? t0 + per_wg : ntiles);
+  const int cur = par ^ 1;
+  int t = fp_next(dom, t0, end, tx, ty, par);
+  if (t >= end) return;  // the whole work-group: nothing to do this step
+  // weights and biases of the four layers into LDS
+  for (int i = threadIdx.x; i < FP_WN / 8; i += blockDim.x) {
+    // layer 0 from its 8-channel packing, layers 1..3 from the forward packing
+    const u16* src = i < FP_W1 / 8 ? wl1 + 8 * i : wf + CL_PACKED + 8 * (i - FP_W1 / 8);
+    *(uint4*)(sw + 8 * i) = *(const uint4*)src;
+  }
+  for (int i = threadIdx.x; i < CL_LAYERS * CL_C; i += blockDim.x)
+    sb[i] = w[(i / CL_C) * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C + i % CL_C];
+  FpTile T = fp_tile(t, tx, ty);
+  FpDom DT = fp_dom(dom, T.d, cur, ms);
+  FpCells pf;
+  fp_load(DT, T, pf);
+  int tn = fp_next(dom, t + 1, end, tx, ty, par);
+  FpTile N = fp_tile(tn < end ? tn : t, tx, ty);
+  FpDom DN = fp_dom(dom, N.d, cur, ms);
+  fp_store(DT, T, pf, f0, core[0]);
+  __syncthreads();
+  double v = 0.0, fl = 0.0;
+  unsigned nsat = 0;
+  int it = 0;
+  for (;;) {
+    // tile t runs the network; the inputs of tile tn are loaded meanwhile, and the
+    // domain descriptor of the tile after it is read (scalar loads, used next iteration)
+    if (tn < end) fp_load(DN, N, pf);  // in flight during layers 0 and 1
+    const int W = DT.W, H = DT.H;
+    fp_layer<FR0, FR1, 2, true>(f0, b1, sw, sb, T.ox, T.oy, W, H, nullptr);
+    const int tnn = tn < end ? fp_next(dom, tn + 1, end, tx, ty, par) : end;
+    const FpTile NN = fp_tile(tnn < end ? tnn : (tn < end ? tn : t), tx, ty);
+    const FpDom DNN = fp_dom(dom, NN.d, cur, ms);
+    __syncthreads();
+    fp_layer<FR1, FR2, 2, false>(b1, b2, sw + FP_W1, sb + CL_C, T.ox, T.oy, W, H, nullptr);
+    float* cnext = core[(it + 1) & 1];
+    if (tn < end) fp_store(DN, N, pf, f0, cnext);
+    __syncthreads();
+    fp_layer<FR2, FR3, 2, false>(b2, b1, sw + FP_W2, sb + 2 * CL_C, T.ox, T.oy, W, H, nullptr);
+    __syncthreads();
+    float* o2 = (float*)b2;
+    fp_layer<FR3, FT, 1, false>(b1, nullptr, sw + FP_W3, sb + 3 * CL_C, T.ox, T.oy, W, H, o2);
+    __syncthreads();
+    const float* cc = core[it & 1];
+    const int e = threadIdx.x;
+    const int x = T.ox + e % FT, y = T.oy + e / FT;
+    if (e < FT * FT && x < W && y < H) {
+      if (o_out) {
+        o_out[((long)y * W + x) * 2] = o2[e * 2];
+        o_out[((long)y * W + x) * 2 + 1] = o2[e * 2 + 1];
+      } else {
+        const float qx = cc[e * 3], qy = cc[e * 3 + 1], hc = cc[e * 3 + 2];
+        const bool rows = y >= 2 && y <= H - 3, cols = x >= 2 && x <= W - 3;
+        if (rows && cols) v += (double)hc;
+        if (rows && x == W - 3) fl += (double)qx;
+        if (rows && x == 1) fl -= (double)qx;
+        if (cols && y == H - 3) fl += (double)qy;
+        if (cols && y == 1) fl -= (double)qy;
+        const long i = (long)y * DT.P + x;
+        if (x >= 1 && x <= W - 3 && rows) {
+          const float ov = o2[e * 2];
+          if (fabsf(ov) > sat) nsat++;
+          else if (!DT.fallback) DT.qx[i] = qx + tanhf(ov) * bound * fabsf(qx);
+        }
+        if (cols && y >= 1 && y <= H - 3) {
+          const float ov = o2[e * 2 + 1];
+          if (fabsf(ov) > sat) nsat++;
+          else if (!DT.fallback) DT.qy[i] = qy + tanhf(ov) * bound * fabsf(qy);
+        }
+      }
+    }
+    const bool last = tn >= end;
+    if (!o_out && (last || N.d != T.d)) {  // flush the mass terms and counts of domain T.d
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        v += __shfl_xor(v, m);
+        fl += __shfl_xor(fl, m);
+        nsat += __shfl_xor(nsat, m);
+      }
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      if (lane == 0) {
+        red[2 * wave] = v;
+        red[2 * wave + 1] = fl;
+        if (nsat) atomicAdd(&ms[T.d].nsat, nsat);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double sv = 0.0, sf = 0.0;
+        for (int k = 0; k < FNW; k++) {
+          sv += red[2 * k];
+          sf += red[2 * k + 1];
+        }
+        atomicAdd(&acc[2 * T.d], sv);
+        atomicAdd(&acc[2 * T.d + 1], sf);
+      }
+      v = fl = 0.0;
+      nsat = 0;
+    }
+    if (last) break;
+    t = tn;
+    T = N;
+    DT = DN;
+    tn = tnn;
+    N = NN;
+    DN = DNN;
+    it++;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Targets of a training sample: the fine domain's current face discharges restricted as
 // in cl_restrict, [pixel][2] fp32, and the mask of trainable faces (the ones cl_correct
 // corrects).
@@ -904,8 +1236,27 @@ CL_WRAP(cl_pack_l1, (const float* w, u16* wl1, hipStream_t st), P(w, wl1), dim3(
 
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
                            const float* w, float bound, float sat, ClMass* ms, double* acc, float* o_out,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(cl_fused, dim3((W + FT - 1) / FT, (H + FT - 1) / FT, ndom), dim3(512), 0, st, dom, par, wf,
-                     wl1, w, bound, sat, ms, acc, o_out);
+                           hipStream_t st, int form) {
+  const int tx = (W + FT - 1) / FT, ty = (H + FT - 1) / FT;
+  if (form == 0) {
+    hipLaunchKernelGGL(cl_fused, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, bound, sat, ms, acc,
+                       o_out);
+    return hipGetLastError();
+  }
+  // persistent: one work-group per CU, contiguous runs of tiles
+  static int cus[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  if (!cus[dev]) {
+    e = hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+  }
+  const int ntiles = tx * ty * ndom;
+  const int per = (ntiles + cus[dev] - 1) / cus[dev];
+  const int groups = (ntiles + per - 1) / per;
+  hipLaunchKernelGGL(cl_fused_p, dim3(groups), dim3(512), 0, st, dom, par, wf, wl1, w, bound, sat, ms, acc, o_out,
+                     tx, ty, ntiles, per);
   return hipGetLastError();
 }

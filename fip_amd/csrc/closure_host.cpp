@@ -140,6 +140,7 @@ struct fip_closure {
   u16* wl1 = nullptr;   // layer 0 in the 8-channel packing of the fused inference kernel
   float* part = nullptr;
   int t = 0;            // Adam steps taken
+  int fused_form = 1;   // launch form of the fused inference kernel (cl_fused_launch)
   // training samples
   int Wc = 0, Hc = 0, cap = 0, ns = 0;
   u16* s_feat = nullptr;     // [cap][Hc][Wc][8]
@@ -479,7 +480,7 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
     const fip_grid& g = fip_internal::grid(ap->sim);
     CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, kBound, ap->sat, ap->ms, ap->acc,
-                         nullptr, st));
+                         nullptr, st, cl->fused_form));
     CHIP(cl_mass_check_launch(g, par, ap->acc, ap->ms, ap->limit, st));
     return FIP_OK;
   });
@@ -525,7 +526,7 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   fip_internal::ens_set_post_hook(ens, [cl, ap, n](hipStream_t st, int par) -> int {
     const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ap->ens, 0));
     CHIP(cl_fused_launch(ap->dom, n, g0.W, g0.H, par, cl->wf, cl->wl1, cl->w, kBound, ap->sat, ap->ms, ap->acc,
-                         nullptr, st));
+                         nullptr, st, cl->fused_form));
     CHIP(cl_mass_check_ens_launch(fip_internal::ens_scn(ap->ens), n, par, ap->acc, ap->ms, ap->limit, st));
     return FIP_OK;
   });
@@ -629,11 +630,18 @@ int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed
   CHIP(hipMemcpy(dom, &hd, sizeof hd, hipMemcpyHostToDevice));
   // par such that par ^ 1 is the current side and the halt flag read is that of no step
   const int par = fip_internal::cur_side(coarse) ^ 1;
-  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, kBound, 1e30f, ms, acc, o, nullptr);
+  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, kBound, 1e30f, ms, acc, o, nullptr, cl->fused_form);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(out, o, spx * 2 * sizeof(float), hipMemcpyDeviceToHost);
   hipFree(bstd); hipFree(o); hipFree(dom); hipFree(ms); hipFree(acc);
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("fused outputs: ") + hipGetErrorString(e));
+  return FIP_OK;
+}
+
+int fip_closure_set_kernel(fip_closure* cl, int form) {
+  if (!cl) return fail(FIP_E_INVALID, "null argument");
+  if (form != 0 && form != 1) return fail(FIP_E_INVALID, "kernel form must be 0 (per tile) or 1 (persistent)");
+  cl->fused_form = form;
   return FIP_OK;
 }
 

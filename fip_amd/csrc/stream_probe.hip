@@ -12,6 +12,10 @@
 //             dense kernel does;
 //   flat      the same byte counts as one flat stream: 16-byte accesses per lane
 //             (4 bytes per lane for the class field), grid-stride;
+//   flat/nt   the flat stream with non-temporal (streaming) stores;
+//   aos       the same cells with h, qx, qy, bed interleaved as one float4 per cell: one
+//             16-byte read and one 16-byte write per cell (bed rewritten) plus the class
+//             byte, 33 B per cell, with plain or non-temporal stores;
 //   copy      a plain 16-byte-per-lane copy of one field (1 read + 1 write).
 // Every access stays inside its allocation: rows are below H, columns are clamped to
 // [0, W - 1], flat indices are below W * H / 4.
@@ -69,6 +73,51 @@ __global__ __launch_bounds__(256) void flat(Fields f, size_t n4) {
     }
 #pragma unroll
     for (int k = 0; k < kOut; k++) reinterpret_cast<float4*>(f.out[k])[i] = s;
+  }
+}
+
+template <bool NT>
+__global__ __launch_bounds__(256) void flat_st(Fields f, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const uchar4 cl = reinterpret_cast<const uchar4*>(f.cls)[i];
+    float4 s = make_float4(cl.x, cl.y, cl.z, cl.w);
+#pragma unroll
+    for (int k = 0; k < kIn; k++) {
+      const float4 v = reinterpret_cast<const float4*>(f.in[k])[i];
+      s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    }
+#pragma unroll
+    for (int k = 0; k < kOut; k++) {
+      float4* d = reinterpret_cast<float4*>(f.out[k]) + i;
+      if (NT) {
+        __builtin_nontemporal_store(s.x, &d->x);
+        __builtin_nontemporal_store(s.y, &d->y);
+        __builtin_nontemporal_store(s.z, &d->z);
+        __builtin_nontemporal_store(s.w, &d->w);
+      } else {
+        *d = s;
+      }
+    }
+  }
+}
+
+// n cells: a[i] = (h, qx, qy, bed), cls[i]; b[i] = (h', qx', qy', bed)
+template <bool NT>
+__global__ __launch_bounds__(256) void aos(const float4* a, const unsigned char* cls, float4* b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float4 v = a[i];
+    const float c = cls[i];
+    const float4 r = make_float4(v.x + c, v.y * c, v.z - c, v.w);
+    if (NT) {
+      __builtin_nontemporal_store(r.x, &b[i].x);
+      __builtin_nontemporal_store(r.y, &b[i].y);
+      __builtin_nontemporal_store(r.z, &b[i].z);
+      __builtin_nontemporal_store(r.w, &b[i].w);
+    } else {
+      b[i] = r;
+    }
   }
 }
 
@@ -137,6 +186,25 @@ int main() {
   for (int blocks : {2048, 8192}) {
     const float ms = time_ms([&] { hipLaunchKernelGGL(flat, dim3(blocks), dim3(256), 0, 0, f, n4); }, reps);
     std::printf("| flat | %d blocks of 256 | %.4f | %.0f |\n", blocks, ms, 29.0 * 4 * n4 / ms / 1e6);
+  }
+  for (int blocks : {2048, 8192}) {
+    const float ms = time_ms([&] { hipLaunchKernelGGL(flat_st<true>, dim3(blocks), dim3(256), 0, 0, f, n4); }, reps);
+    std::printf("| flat/nt | %d blocks of 256 | %.4f | %.0f |\n", blocks, ms, 29.0 * 4 * n4 / ms / 1e6);
+  }
+  {
+    const size_t nc = (size_t)kW * kH;
+    float4 *a = nullptr, *b = nullptr;
+    CK(hipMalloc(&a, nc * sizeof(float4)));
+    CK(hipMalloc(&b, nc * sizeof(float4)));
+    CK(hipMemset(a, 0, nc * sizeof(float4)));
+    for (int blocks : {2048, 8192}) {
+      float ms = time_ms([&] { hipLaunchKernelGGL(aos<false>, dim3(blocks), dim3(256), 0, 0, a, f.cls, b, nc); }, reps);
+      std::printf("| aos | %d blocks of 256 | %.4f | %.0f (33 B per cell moved) |\n", blocks, ms, bytes / ms / 1e6);
+      ms = time_ms([&] { hipLaunchKernelGGL(aos<true>, dim3(blocks), dim3(256), 0, 0, a, f.cls, b, nc); }, reps);
+      std::printf("| aos/nt | %d blocks of 256 | %.4f | %.0f (33 B per cell moved) |\n", blocks, ms, bytes / ms / 1e6);
+    }
+    CK(hipFree(a));
+    CK(hipFree(b));
   }
   const float ms = time_ms([&] {
     hipLaunchKernelGGL(copy, dim3(8192), dim3(256), 0, 0, reinterpret_cast<const float4*>(f.in[0]),

@@ -27,7 +27,7 @@ CLOSURE_EXPORTS = ["fip_restrict", "fip_closure_create", "fip_closure_destroy", 
                    "fip_closure_train", "fip_closure_eval", "fip_closure_attach",
                    "fip_closure_attach_ensemble", "fip_closure_detach", "fip_closure_status",
                    "fip_closure_forward", "fip_closure_backward", "fip_closure_fused_outputs",
-                   "fip_mfma_probe"]
+                   "fip_closure_set_kernel", "fip_mfma_probe"]
 CL_PARAMS = 4 * (9 * 32 * 32 + 32)
 ENS_EXPORTS = ["fip_ens_create", "fip_ens_destroy", "fip_ens_set_bcs", "fip_ens_set_run",
                "fip_ens_simulate", "fip_ens_get_time", "fip_ens_download", "fip_ens_set_launch",
@@ -131,6 +131,7 @@ def lib():
         L.fip_closure_backward.argtypes = [vp, fp, C.c_int, C.c_int, C.c_int, fp, fp,
                                            C.POINTER(C.c_ubyte), C.c_float, fp, dp]
         L.fip_closure_fused_outputs.argtypes = [vp, vp, fp, C.c_float, fp]
+        L.fip_closure_set_kernel.argtypes = [vp, C.c_int]
         L.fip_mfma_probe.argtypes = [C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), fp]
         L.fip_tune.argtypes = [C.POINTER(FipDesc), C.POINTER(FipBC), C.c_int, C.c_int,
                                C.POINTER(FipTune)]
@@ -543,6 +544,10 @@ class Closure:
         out = np.zeros((N, H, W, 2), np.float32)
         _check(lib().fip_closure_forward(self.c, _f32p(f), N, H, W, _f32p(out)))
         return out
+
+    def set_kernel(self, form: int):
+        """Launch form of the fused inference kernel: 1 persistent (default), 0 per tile."""
+        _check(lib().fip_closure_set_kernel(self.c, int(form)))
 
     def fused_outputs(self, coarse: "Sim", bed_std, zref):
         b = np.ascontiguousarray(bed_std, dtype=np.float32)

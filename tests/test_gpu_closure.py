@@ -230,3 +230,37 @@ def test_fused_step_matches_layerwise_forward():
     for x in (a, b):
         x.close()
     cl.close()
+
+
+@pytest.mark.parametrize("shape", [(96, 64), (50, 37)])
+def test_persistent_fused_kernel_equals_tile_form(shape):
+    """The persistent fused kernel (weights in LDS, prefetched tiles) gives the tile
+    form's outputs and corrections bit for bit, on a multiple and a non-multiple of the
+    16 x 16 tile."""
+    from fip_amd.sim import Closure, Sim
+    Wc, Hc = shape
+    fine, coarse, bstd, zref = fine_coarse_case(Wc * 4, Hc * 4, 1.0, 4, scenario=7, wet=0.5)
+    rng = np.random.default_rng(5)
+    p = rand_params(rng, 0.3)
+    cls, sims, outs = [], [], []
+    for form in (0, 1):
+        cl = Closure(Wc, Hc, 0, seed=2)
+        cl.params = p
+        cl.set_kernel(form)
+        s = Sim(coarse, mode="dense")
+        s.set_run(1e9)
+        s.simulate(20)
+        outs.append(cl.fused_outputs(s, bstd, zref))
+        cl.attach(s, bstd, zref, sat=3.0)
+        s.simulate(20)
+        cls.append(cl)
+        sims.append(s)
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    for k in ("h", "qx", "qy"):
+        assert np.array_equal(sims[0].field(k).view(np.uint32), sims[1].field(k).view(np.uint32)), k
+    st0, st1 = cls[0].status(sims[0]), cls[1].status(sims[1])
+    assert st0["saturated_faces"] == st1["saturated_faces"] > 0
+    assert st0["fallback"] == st1["fallback"] == 0
+    assert st1["max_rel_mass_error"] < 1e-5, st1
+    for x in sims + cls:
+        x.close()
