@@ -111,6 +111,45 @@ def run_ensemble(variant, steps, warmup, nscn=64):
     return out
 
 
+def full_run(variant, rows, chunk=5000, max_steps=0):
+    """The 5667 x 6500 workload over its whole 12 h from the dry start in masked mode:
+    steps, wall time, effective rate (W*H per step) and the active-tile fraction sampled
+    every `chunk` steps. Returns a dict for a JSON line."""
+    from fip_amd.sim import Sim
+    from fip_amd.workloads import config1
+    case = config1()
+    valid = case.bed != np.float32(case.no_data)
+    s = Sim(case, mode="masked", variant=variant, tile_rows=rows)
+    s.set_run(case.duration)
+    samples = []
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        ev, done = s.simulate(chunk)
+        steps += done
+        st = s.stats()
+        t, dt, it = s.time()
+        samples.append((round(t, 1), round(st["tiles_computed"] / st["tiles_total"], 4)))
+        print(f"full run: {steps} steps, t = {t:.0f} s, dt = {dt:.4f} s, active tiles "
+              f"{st['tiles_computed']}/{st['tiles_total']}", file=sys.stderr, flush=True)
+        if ev == 1 or done < chunk or (max_steps and steps >= max_steps):
+            break
+    wall = time.perf_counter() - t0
+    h = s.field("h")
+    wet = float(((h > 0.01) & valid).sum() / valid.sum())
+    vol = float(h.astype(np.float64)[valid].sum() * case.dx * case.dx)
+    out = {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q hydrograph "
+                       "0->600->0 m3/s over 12 h, right open, dry start, masked mode",
+           "variant": variant, "tile_rows": rows, "simulated_s": round(s.time()[0], 2), "steps": steps,
+           "wall_s": round(wall, 2), "ms_per_step": round(wall / steps * 1e3, 4),
+           "gcells_effective": round(case.W * case.H * steps / wall / 1e9, 2),
+           "simulated_h_per_wall_s": round(s.time()[0] / 3600.0 / wall, 3),
+           "final_wet_fraction_of_valid": round(wet, 4), "final_volume_m3": round(vol, 1),
+           "active_tile_fraction": samples}
+    s.close()
+    return out
+
+
 def measured_traffic(variant, rows):
     """HBM bytes per stencil launch from the committed rocprofv3 --pmc summary of this
     workload (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
@@ -169,11 +208,19 @@ def main():
     ap.add_argument("--closure-test", type=int, default=8, help="held-out scenarios")
     ap.add_argument("--closure-duration", type=float, default=1800.0)
     ap.add_argument("--closure-steps", type=int, default=4000, help="Adam steps")
+    ap.add_argument("--full-run", action="store_true",
+                    help="run the 5667x6500 workload over its whole 12 h from the dry start (masked) "
+                         "and print that leg's JSON line only")
+    ap.add_argument("--full-run-steps", type=int, default=0,
+                    help="stop the full run after this many steps (0: the whole 12 h)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run 1 GiB device copies (counter calibration) and exit")
     args = ap.parse_args()
     if args.calibrate:
         calibrate_copy()
+        return
+    if args.full_run:
+        print(json.dumps(full_run(args.variant, args.rows, max_steps=args.full_run_steps)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the single-domain step does not shard "
