@@ -206,3 +206,43 @@ def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1
     res["eval"] = evaluate(cl, range(1000, 1000 + n_test), duration, W=W, log=log)
     cl.close()
     return res
+
+
+def time_ensemble_step(steps=300, n=8, W=1024, warm=200, sat=1e30, seed=11, log=print):
+    """Per-step wall time of the held-out coarse twins (n scenarios of (W/4)^2) as a plain
+    ensemble and with a closure of seeded random weights attached (every face corrected:
+    sat = 1e30). Returns (plain_us, learned_us)."""
+    from .sim import Closure, Ensemble
+    fines = [fine_case(1000 + s, W=W, H=W, duration=1800.0) for s in range(n)]
+    pairs = [coarse_of(f) for f in fines]
+    coarses = [p[0] for p in pairs]
+    bstd = np.stack([p[1] for p in pairs])
+    zref = np.array([p[2] for p in pairs], np.float32)
+    cl = Closure(W // F, W // F, max_samples=0, seed=seed)
+    out = []
+    for label, closure in (("plain", None), ("learned", cl)):
+        e = Ensemble(coarses, mode="dense", variant="fast", tile_rows=32)
+        if closure is not None:
+            closure.attach_ensemble(e, bstd, zref, sat=sat)
+        e.set_run([1e9] * n)
+        e.simulate(warm)
+        t0 = time.perf_counter()
+        e.simulate(steps)
+        e.time(0)
+        us = (time.perf_counter() - t0) / steps * 1e6
+        log(f"{label}: {us:.1f} us per step")
+        out.append(us)
+        e.close()
+    cl.close()
+    return tuple(out)
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="time the coarse ensemble step with and without the closure")
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--scenarios", type=int, default=8)
+    ap.add_argument("--fine", type=int, default=1024, help="fine grid size (the coarse one is a quarter)")
+    ap.add_argument("--warm", type=int, default=200)
+    a = ap.parse_args()
+    time_ensemble_step(a.steps, a.scenarios, a.fine, a.warm)
