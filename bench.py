@@ -288,7 +288,7 @@ def main():
         rmse_ok = ev["learned_depth_rmse_m"] <= 0.02
         out["closure"] = {
             "workload": (f"{args.closure_train} training + {args.closure_test} held-out scenarios, fine "
-                         f"1024^2 dx=1 m (ridged terrain, z inflow), coarse 256^2 dx=4 m, "
+                         f"1024^2 dx=1 m (ridged terrain, discharge inflow), coarse 256^2 dx=4 m, "
                          f"{args.closure_duration:.0f} s simulated, fixed dt 0.05/0.2 s"),
             "speedup_learned_vs_fine": ev["speedup_learned_vs_fine"],
             "speedup_at_rmse_le_2cm": ev["speedup_learned_vs_fine"] if rmse_ok else None,
