@@ -49,9 +49,11 @@ struct ClDom {
 };
 #define CL_PACKED_L1 (2 * 3 * 64 * 8)   // bf16 elements of the 8-channel packing of layer 0
 
+// Network outputs o ([ndom][H][W][2]) of every cell of ndom domains of one size, and their
+// mass-balance sums into acc (2 per domain) when acc != NULL. form 1: persistent kernel,
+// 0: one work-group per tile.
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                           const float* w, float bound, float sat, ClMass* ms, double* acc, float* o_out,
-                           hipStream_t st, int form = 1);
+                           const float* w, double* acc, float* o, hipStream_t st, int form = 1);
 hipError_t cl_pack_l1_launch(const float* w, u16* wl1, hipStream_t st);
 hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
                           u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,
@@ -62,8 +64,6 @@ hipError_t cl_restrict_flux_launch(fip_grid f, int fcur, int F, int Wc, int Hc, 
                                    hipStream_t st);
 hipError_t cl_flux_launch(fip_grid g, int cur, float* q, hipStream_t st);
 hipError_t cl_features_launch(ClFeat s, u16* out, int cstride, const fip_ctl* ctl, int par, hipStream_t st);
-hipError_t cl_features_ens_launch(const fip_ens_scn* scn, const float* bstd, const float* zref, int par,
-                                  u16* out, int W, int H, int n, hipStream_t st);
 hipError_t cl_gather_launch(const u16* samples, long sample_elems, int W, const int* origin, int ps, int ph,
                             u16* out, int nb, hipStream_t st);
 hipError_t cl_gather_targets_launch(const float* qp, const float* qt, const uint8_t* mask, long sample_pixels,
@@ -80,10 +80,8 @@ hipError_t cl_adam_launch(float* w, float* m, float* v, const float* g, int nlay
                           float b2, float eps, float c1, float c2, hipStream_t st);
 hipError_t cl_correct_launch(fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms,
                              hipStream_t st);
-hipError_t cl_mass_sum_launch(fip_grid g, int par, double* acc, hipStream_t st);
 hipError_t cl_mass_check_launch(fip_grid g, int par, double* acc, ClMass* ms, double limit, hipStream_t st);
 hipError_t cl_correct_ens_launch(const fip_ens_scn* scn, int par, const float* o, float bound, float sat,
                                  ClMass* ms, int W, int H, int n, hipStream_t st);
-hipError_t cl_mass_sum_ens_launch(const fip_ens_scn* scn, int par, double* acc, int W, int H, int n, hipStream_t st);
 hipError_t cl_mass_check_ens_launch(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
                                     hipStream_t st);

@@ -416,6 +416,35 @@ extern "C" __global__ void cl_adam(float* w, float* m, float* v, const float* g,
   w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// The correction of cell (x, y)'s east and north faces by the network output o ([H][W][2]):
+// on interior faces q += tanh(o) * bound * |q| unless |o| > sat (counted, the face keeps
+// the plain step) or the domain fell back (off). Returns the saturated faces.
+__device__ __forceinline__ unsigned correct_cell(const fip_grid& g, int cur, const float* o, int x, int y,
+                                                 float bound, float sat, bool off) {
+  const long p = (long)y * g.W + x, i = (long)y * g.P + x;
+  unsigned nsat = 0;
+  if (x >= 1 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) {
+    const float ox = o[p * 2];
+    const float q = g.qx[cur][i];
+    if (fabsf(ox) > sat) nsat++;
+    else if (!off) g.qx[cur][i] = q + tanhf(ox) * bound * fabsf(q);
+  }
+  if (x >= 2 && x <= g.W - 3 && y >= 1 && y <= g.H - 3) {
+    const float oy = o[p * 2 + 1];
+    const float q = g.qy[cur][i];
+    if (fabsf(oy) > sat) nsat++;
+    else if (!off) g.qy[cur][i] = q + tanhf(oy) * bound * fabsf(q);
+  }
+  return nsat;
+}
+
+// one atomic per wavefront (every lane of the wavefront must call it)
+__device__ __forceinline__ void add_wave_count(unsigned* dst, unsigned n) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m);
+  if ((threadIdx.x & 63) == 0 && n) atomicAdd(dst, n);
+}
+
 // ---------------------------------------------------------------------------------------
 // Correction of a coarse domain's current face discharges by the network output o (fp32
 // [pixel][2]): on interior faces (qx: columns 1..W-3, rows 2..H-3; qy: columns 2..W-3,
@@ -425,35 +454,22 @@ extern "C" __global__ void cl_adam(float* w, float* m, float* v, const float* g,
 // water between two cells of the domain, so the volume is conserved by construction.
 extern "C" __global__ void cl_correct(fip_grid g, int par, const float* o, float bound, float sat,
                                       ClMass* ms) {
-  if (g.ctl->halt_req[par ^ 1] || ms->fallback) return;
+  if (g.ctl->halt_req[par ^ 1]) return;
   const int cur = par ^ 1;
+  const bool off = ms->fallback != 0;
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   unsigned nsat = 0;
-  if (x < g.W && y < g.H) {
-    const long p = (long)y * g.W + x, i = (long)y * g.P + x;
-    if (x >= 1 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) {
-      const float ox = o[p * 2];
-      const float q = g.qx[cur][i];
-      if (fabsf(ox) <= sat) g.qx[cur][i] = q + tanhf(ox) * bound * fabsf(q);
-      else nsat++;
-    }
-    if (x >= 2 && x <= g.W - 3 && y >= 1 && y <= g.H - 3) {
-      const float oy = o[p * 2 + 1];
-      const float q = g.qy[cur][i];
-      if (fabsf(oy) <= sat) g.qy[cur][i] = q + tanhf(oy) * bound * fabsf(q);
-      else nsat++;
-    }
-  }
-  if (nsat) atomicAdd(&ms->nsat, nsat);
+  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o, x, y, bound, sat, off);
+  add_wave_count(&ms->nsat, nsat);
 }
 
 // Mass balance of a coarse domain without no_data cells, checked after each step before
 // its correction. The continuity update telescopes over the box of cells
 // [2, W-3] x [2, H-3], which no boundary condition writes, so over the step just taken
 //   V(t) - V(t - dt) = -dt dx (sum_j qx(W-3, j) - qx(1, j) + sum_i qy(i, H-3) - qy(i, 1))
-// with the face discharges the step computed (stored and not yet corrected). cl_mass_sum
-// adds the box volume and the bracket into acc[2] (fp64); cl_mass_check compares and sets
-// the fallback flag when the relative error exceeds `limit`.
+// with the face discharges the step computed (stored and not yet corrected). The fused
+// kernels add the box volume and the bracket into acc[2] (fp64); cl_mass_check compares
+// and sets the fallback flag when the relative error exceeds `limit`.
 __device__ __forceinline__ void mass_terms(const fip_grid& g, int cur, int x, int y, double& v, double& f) {
   v = 0.0;
   f = 0.0;
@@ -502,58 +518,25 @@ __device__ __forceinline__ void mass_check(const fip_grid& g, double* acc, ClMas
   acc[1] = 0.0;
 }
 
-extern "C" __global__ void cl_mass_sum(fip_grid g, int par, double* acc) {
-  if (g.ctl->halt_req[par ^ 1]) return;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  double v, f;
-  mass_terms(g, par ^ 1, x, y, v, f);
-  __shared__ double sh[2 * 256];
-  block_sum2(v, f, acc, sh);
-}
-
 extern "C" __global__ void cl_mass_check(fip_grid g, int par, double* acc, ClMass* ms, double limit) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (g.ctl->halt_req[par ^ 1]) return;
   mass_check(g, acc, *ms, limit);
 }
 
-// Ensemble forms of cl_correct / cl_mass_sum / cl_mass_check: blockIdx.z (or the thread
+// Ensemble forms of cl_correct / cl_mass_check: blockIdx.z (or the thread
 // of cl_mass_check_ens) = scenario; o holds one W x H image per scenario.
 extern "C" __global__ void cl_correct_ens(const fip_ens_scn* scn, int par, const float* o, float bound,
                                           float sat, ClMass* ms) {
   const int s = blockIdx.z;
   const fip_grid& g = scn[s].g;
-  if (g.ctl->halt_req[par ^ 1] || ms[s].fallback) return;
-  const int cur = par ^ 1;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= g.W || y >= g.H) return;
-  const float* os = o + (long)s * g.W * g.H * 2;
-  const long p = (long)y * g.W + x, i = (long)y * g.P + x;
-  unsigned nsat = 0;
-  if (x >= 1 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) {
-    const float ox = os[p * 2];
-    const float q = g.qx[cur][i];
-    if (fabsf(ox) <= sat) g.qx[cur][i] = q + tanhf(ox) * bound * fabsf(q);
-    else nsat++;
-  }
-  if (x >= 2 && x <= g.W - 3 && y >= 1 && y <= g.H - 3) {
-    const float oy = os[p * 2 + 1];
-    const float q = g.qy[cur][i];
-    if (fabsf(oy) <= sat) g.qy[cur][i] = q + tanhf(oy) * bound * fabsf(q);
-    else nsat++;
-  }
-  if (nsat) atomicAdd(&ms[s].nsat, nsat);
-}
-
-extern "C" __global__ void cl_mass_sum_ens(const fip_ens_scn* scn, int par, double* acc) {
-  const int s = blockIdx.z;
-  const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
+  const int cur = par ^ 1;
+  const bool off = ms[s].fallback != 0;
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  double v, f;
-  mass_terms(g, par ^ 1, x, y, v, f);
-  __shared__ double sh[2 * 256];
-  block_sum2(v, f, acc + 2 * s, sh);
+  unsigned nsat = 0;
+  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o + (long)s * g.W * g.H * 2, x, y, bound, sat, off);
+  add_wave_count(&ms[s].nsat, nsat);
 }
 
 extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms,
@@ -568,12 +551,13 @@ extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int 
 // ---------------------------------------------------------------------------------------
 // Fused inference step of the closure for 16 x 16 coarse cells per work-group: features,
 // the four convolutions (activations kept in LDS, the halo recomputed: layer l is
-// evaluated on (16 + 2 (4 - l))^2 cells), the bounded correction of the cells' east/north
-// face discharges and the mass-balance sums, in one launch. blockIdx.z = domain.
-// Activations outside the domain are 0, as the zero padding of the layer-by-layer form.
-// The mass sums use the face discharges before correction (each face is owned by one
-// cell, so no work-group reads a face another one corrects); a fallback raised by the
-// mass check after this step takes effect from the next step.
+// evaluated on (16 + 2 (4 - l))^2 cells), the network outputs of the cells and the
+// mass-balance sums, in one launch. blockIdx.z = domain. Activations outside the domain
+// are 0, as the zero padding of the layer-by-layer form. The correction itself is a
+// separate launch (cl_correct / cl_correct_ens): a work-group reads the discharges of a
+// 4-cell halo around its tile, which other work-groups own, so correcting in place here
+// would let a work-group read a neighbour's corrected faces or not, depending on timing.
+// A fallback raised by the mass check after a step takes effect from the next step.
 #define FT 16                 // output tile
 #define FR0 (FT + 8)          // feature region
 #define FR1 (FT + 6)
@@ -673,8 +657,8 @@ __device__ __forceinline__ void fused_layer(const u16* in, u16* out, const u16* 
 }
 
 extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, int par, const u16* wf,
-                                                          const u16* wl1, const float* w, float bound,
-                                                          float sat, ClMass* ms, double* acc, float* o_out) {
+                                                          const u16* wl1, const float* w, double* acc,
+                                                          float* o_out) {
   __shared__ __attribute__((aligned(16))) u16 f0[FR0 * FR0 * 8];
   __shared__ __attribute__((aligned(16))) u16 b1[FR1 * FR1 * FPS];
   __shared__ __attribute__((aligned(16))) u16 b2[FR2 * FR2 * FPS];
@@ -718,42 +702,26 @@ extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, 
   fused_layer<FR3, FT, 1, false>(b1, nullptr, wf + 3 * CL_PACKED, w + 3 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C,
                                  ox, oy, W, H, o2);
   __syncthreads();
-  if (o_out) {  // network outputs only ([H][W][2]), no correction
-    for (int e = threadIdx.x; e < FT * FT; e += blockDim.x) {
-      const int x = ox + e % FT, y = oy + e / FT;
-      if (x < W && y < H) {
-        o_out[((long)y * W + x) * 2] = o2[e * 2];
-        o_out[((long)y * W + x) * 2 + 1] = o2[e * 2 + 1];
-      }
-    }
-    return;
-  }
-  // correction of the tile's faces and the mass-balance sums (pre-correction values)
-  const bool off = ms[d].fallback != 0;
+  // the outputs of the tile's cells into o (one [H][W][2] image per domain) and the
+  // mass-balance sums (acc != NULL) of the uncorrected state
+  float* od = o_out + (long)d * W * H * 2;
   double v = 0.0, fl = 0.0;
-  unsigned nsat = 0;
   for (int e = threadIdx.x; e < FT * FT; e += blockDim.x) {
     const int x = ox + e % FT, y = oy + e / FT;
     if (x >= W || y >= H) continue;
-    const long i = (long)y * g.P + x;
-    double mv, mf;
-    mass_terms(g, cur, x, y, mv, mf);
-    v += mv;
-    fl += mf;
-    if (x >= 1 && x <= W - 3 && y >= 2 && y <= H - 3) {
-      const float q = g.qx[cur][i], ov = o2[e * 2];
-      if (fabsf(ov) > sat) nsat++;
-      else if (!off) g.qx[cur][i] = q + tanhf(ov) * bound * fabsf(q);
-    }
-    if (x >= 2 && x <= W - 3 && y >= 1 && y <= H - 3) {
-      const float q = g.qy[cur][i], ov = o2[e * 2 + 1];
-      if (fabsf(ov) > sat) nsat++;
-      else if (!off) g.qy[cur][i] = q + tanhf(ov) * bound * fabsf(q);
+    od[((long)y * W + x) * 2] = o2[e * 2];
+    od[((long)y * W + x) * 2 + 1] = o2[e * 2 + 1];
+    if (acc) {
+      double mv, mf;
+      mass_terms(g, cur, x, y, mv, mf);
+      v += mv;
+      fl += mf;
     }
   }
-  __syncthreads();
-  block_sum2(v, fl, acc + 2 * d, (double*)f0);  // the feature buffer (>= 2 x 512 doubles) is free by now
-  if (nsat) atomicAdd(&ms[d].nsat, nsat);
+  if (acc) {
+    __syncthreads();
+    block_sum2(v, fl, acc + 2 * d, (double*)f0);  // the feature buffer (>= 2 x 512 doubles) is free by now
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -765,12 +733,11 @@ extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, 
 //     by every wavefront of every tile;
 //   * the input cells of tile t+1 (bed, h, qx, qy, subgrid std, kSt) are loaded into
 //     registers while tile t runs layer 0, and written to the feature buffer after layer 1;
-//   * the correction takes the pre-correction discharges and depths of its 16 x 16 cells
-//     from LDS (double-buffered), so it issues stores only;
-//   * the mass terms are summed per thread over the work-group's tiles of one domain and
-//     reduced once per domain. Each box-boundary discharge is added by the thread of the
-//     cell that owns the face, from the pre-correction values, so no thread reads a face
-//     that another one corrects.
+//   * the mass terms take the discharges and depths of the tile's 16 x 16 cells from LDS
+//     (double-buffered); they are summed per thread over the work-group's tiles of one
+//     domain and reduced once per domain. Each box-boundary discharge is added by the
+//     thread of the cell that owns the face.
+// Like cl_fused it writes the network outputs; cl_correct(_ens) applies them.
 // Four barriers per tile. LDS: 52 KB of weights + 80 KB of activations (one work-group
 // per CU, 8 wavefronts).
 #define FP_PIX (FR0 * FR0)                    // 576 feature pixels per tile
@@ -904,16 +871,15 @@ __device__ __forceinline__ int fp_next(const ClDom* __restrict__ dom, int t, int
 
 // What a tile needs of its domain, read once (scalar loads) one tile ahead of its use.
 struct FpDom {
-  const float *h, *bed, *kst, *bstd;
-  float *qx, *qy;
+  const float *h, *qx, *qy, *bed, *kst, *bstd;
   float kst_u, zref;
-  int W, H, P, fallback;
+  int W, H, P;
 };
 
-__device__ __forceinline__ FpDom fp_dom(const ClDom* __restrict__ dom, int d, int cur, const ClMass* ms) {
+__device__ __forceinline__ FpDom fp_dom(const ClDom* __restrict__ dom, int d, int cur) {
   const ClDom& D = dom[d];
-  return FpDom{D.g.h[cur], D.g.bed, D.g.kst, D.bstd, D.g.qx[cur], D.g.qy[cur], D.g.kst_uniform, D.zref,
-               D.g.W, D.g.H, D.g.P, uni(ms[d].fallback)};
+  return FpDom{D.g.h[cur], D.g.qx[cur], D.g.qy[cur], D.g.bed, D.g.kst, D.bstd, D.g.kst_uniform, D.zref,
+               D.g.W, D.g.H, D.g.P};
 }
 
 struct FpCells {  // the raw inputs of this thread's feature pixels of one tile
@@ -969,8 +935,8 @@ __device__ __forceinline__ void fp_store(const FpDom& D, FpTile T, const FpCells
 }
 
 extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_eu(2, 2))) void cl_fused_p(
-    const ClDom* __restrict__ dom, int par, const u16* wf, const u16* wl1, const float* w, float bound, float sat,
-    ClMass* ms, double* acc, float* o_out, int tx, int ty, int ntiles, int per_wg) {
+    const ClDom* __restrict__ dom, int par, const u16* wf, const u16* wl1, const float* w, double* acc,
+    float* o_out, int tx, int ty, int ntiles, int per_wg) {
   __shared__ __attribute__((aligned(16))) u16 sw[FP_WN];
   __shared__ float sb[CL_LAYERS * CL_C];
   __shared__ __attribute__((aligned(16))) u16 f0[FR0 * FR0 * 8];
@@ -992,16 +958,15 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
   for (int i = threadIdx.x; i < CL_LAYERS * CL_C; i += blockDim.x)
     sb[i] = w[(i / CL_C) * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C + i % CL_C];
   FpTile T = fp_tile(t, tx, ty);
-  FpDom DT = fp_dom(dom, T.d, cur, ms);
+  FpDom DT = fp_dom(dom, T.d, cur);
   FpCells pf;
   fp_load(DT, T, pf);
   int tn = fp_next(dom, t + 1, end, tx, ty, par);
   FpTile N = fp_tile(tn < end ? tn : t, tx, ty);
-  FpDom DN = fp_dom(dom, N.d, cur, ms);
+  FpDom DN = fp_dom(dom, N.d, cur);
   fp_store(DT, T, pf, f0, core[0]);
   __syncthreads();
   double v = 0.0, fl = 0.0;
-  unsigned nsat = 0;
   int it = 0;
   for (;;) {
     // tile t runs the network; the inputs of tile tn are loaded meanwhile, and the
@@ -1011,7 +976,7 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
     fp_layer<FR0, FR1, 2, true>(f0, b1, sw, sb, T.ox, T.oy, W, H, nullptr);
     const int tnn = tn < end ? fp_next(dom, tn + 1, end, tx, ty, par) : end;
     const FpTile NN = fp_tile(tnn < end ? tnn : (tn < end ? tn : t), tx, ty);
-    const FpDom DNN = fp_dom(dom, NN.d, cur, ms);
+    const FpDom DNN = fp_dom(dom, NN.d, cur);
     __syncthreads();
     fp_layer<FR1, FR2, 2, false>(b1, b2, sw + FP_W1, sb + CL_C, T.ox, T.oy, W, H, nullptr);
     float* cnext = core[(it + 1) & 1];
@@ -1026,10 +991,10 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
     const int e = threadIdx.x;
     const int x = T.ox + e % FT, y = T.oy + e / FT;
     if (e < FT * FT && x < W && y < H) {
-      if (o_out) {
-        o_out[((long)y * W + x) * 2] = o2[e * 2];
-        o_out[((long)y * W + x) * 2 + 1] = o2[e * 2 + 1];
-      } else {
+      float* od = o_out + ((long)T.d * W * H + (long)y * W + x) * 2;
+      od[0] = o2[e * 2];
+      od[1] = o2[e * 2 + 1];
+      if (acc) {
         const float qx = cc[e * 3], qy = cc[e * 3 + 1], hc = cc[e * 3 + 2];
         const bool rows = y >= 2 && y <= H - 3, cols = x >= 2 && x <= W - 3;
         if (rows && cols) v += (double)hc;
@@ -1037,32 +1002,19 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
         if (rows && x == 1) fl -= (double)qx;
         if (cols && y == H - 3) fl += (double)qy;
         if (cols && y == 1) fl -= (double)qy;
-        const long i = (long)y * DT.P + x;
-        if (x >= 1 && x <= W - 3 && rows) {
-          const float ov = o2[e * 2];
-          if (fabsf(ov) > sat) nsat++;
-          else if (!DT.fallback) DT.qx[i] = qx + tanhf(ov) * bound * fabsf(qx);
-        }
-        if (cols && y >= 1 && y <= H - 3) {
-          const float ov = o2[e * 2 + 1];
-          if (fabsf(ov) > sat) nsat++;
-          else if (!DT.fallback) DT.qy[i] = qy + tanhf(ov) * bound * fabsf(qy);
-        }
       }
     }
     const bool last = tn >= end;
-    if (!o_out && (last || N.d != T.d)) {  // flush the mass terms and counts of domain T.d
+    if (acc && (last || N.d != T.d)) {  // flush the mass terms of domain T.d
 #pragma unroll
       for (int m = 32; m >= 1; m >>= 1) {
         v += __shfl_xor(v, m);
         fl += __shfl_xor(fl, m);
-        nsat += __shfl_xor(nsat, m);
       }
       const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
       if (lane == 0) {
         red[2 * wave] = v;
         red[2 * wave + 1] = fl;
-        if (nsat) atomicAdd(&ms[T.d].nsat, nsat);
       }
       __syncthreads();
       if (threadIdx.x == 0) {
@@ -1075,7 +1027,6 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
         atomicAdd(&acc[2 * T.d + 1], sf);
       }
       v = fl = 0.0;
-      nsat = 0;
     }
     if (last) break;
     t = tn;
@@ -1138,24 +1089,6 @@ extern "C" __global__ void cl_feat_from_float(const float* f, long npix, u16* ou
   for (int k = 0; k < CL_C; k++) out[p * CL_C + k] = k < CL_FEAT ? f2bf(f[p * CL_FEAT + k]) : (u16)0;
 }
 
-// Features of every scenario of an ensemble (blockIdx.z = scenario) from its current side,
-// NHWC bf16 with 32 channels, one image per scenario (all of one size).
-extern "C" __global__ void cl_features_ens(const fip_ens_scn* scn, const float* bstd, const float* zref,
-                                           int par, u16* out) {
-  const int s = blockIdx.z;
-  const fip_grid& g = scn[s].g;
-  if (g.ctl->halt_req[par ^ 1]) return;
-  const int cur = par ^ 1;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= g.W || y >= g.H) return;
-  ClFeat f{g.h[cur], g.qx[cur], g.qy[cur], g.bed, g.kst, bstd + (long)s * g.W * g.H, g.kst_uniform,
-           zref[s], g.W, g.H, g.P};
-  float v[CL_FEAT];
-  feat_pixel(f, x, y, v);
-  u16* p = out + (((long)s * g.H + y) * g.W + x) * CL_C;
-  for (int k = 0; k < CL_C; k++) p[k] = k < CL_FEAT ? f2bf(v[k]) : (u16)0;
-}
-
 // ---------------------------------------------------------------------------------------
 // launchers (host side of this translation unit)
 hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
@@ -1192,9 +1125,6 @@ CL_WRAP(cl_flux, (fip_grid g, int cur, float* q, hipStream_t st), P(g, cur, q), 
         dim3(256))
 CL_WRAP(cl_features, (ClFeat s, u16* out, int cstride, const fip_ctl* ctl, int par, hipStream_t st),
         P(s, out, cstride, ctl, par), dim3((s.W + 255) / 256, s.H), dim3(256))
-CL_WRAP(cl_features_ens, (const fip_ens_scn* scn, const float* bstd, const float* zref, int par, u16* out, int W,
-                          int H, int n, hipStream_t st),
-        P(scn, bstd, zref, par, out), dim3((W + 255) / 256, H, n), dim3(256))
 CL_WRAP(cl_gather, (const u16* samples, long sample_elems, int W, const int* origin, int ps, int ph, u16* out,
                     int nb, hipStream_t st),
         P(samples, sample_elems, W, origin, ps, out), dim3((ps + 127) / 128, ph, nb), dim3(128))
@@ -1219,15 +1149,11 @@ CL_WRAP(cl_adam, (float* w, float* m, float* v, const float* g, int nlayers, flo
         P(w, m, v, g, nlayers, lr, b1, b2, eps, c1, c2), dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))
 CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms, hipStream_t st),
         P(g, par, o, bound, sat, ms), dim3((g.W + 255) / 256, g.H), dim3(256))
-CL_WRAP(cl_mass_sum, (fip_grid g, int par, double* acc, hipStream_t st), P(g, par, acc),
-        dim3((g.W + 255) / 256, g.H), dim3(256))
 CL_WRAP(cl_mass_check, (fip_grid g, int par, double* acc, ClMass* ms, double limit, hipStream_t st),
         P(g, par, acc, ms, limit), dim3(1), dim3(64))
 CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, int par, const float* o, float bound, float sat, ClMass* ms, int W,
                          int H, int n, hipStream_t st),
         P(scn, par, o, bound, sat, ms), dim3((W + 255) / 256, H, n), dim3(256))
-CL_WRAP(cl_mass_sum_ens, (const fip_ens_scn* scn, int par, double* acc, int W, int H, int n, hipStream_t st),
-        P(scn, par, acc), dim3((W + 255) / 256, H, n), dim3(256))
 CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
                             hipStream_t st),
         P(scn, n, par, acc, ms, limit), dim3((n + 63) / 64), dim3(64))
@@ -1235,12 +1161,10 @@ CL_WRAP(cl_pack_l1, (const float* w, u16* wl1, hipStream_t st), P(w, wl1), dim3(
         dim3(256))
 
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                           const float* w, float bound, float sat, ClMass* ms, double* acc, float* o_out,
-                           hipStream_t st, int form) {
+                           const float* w, double* acc, float* o, hipStream_t st, int form) {
   const int tx = (W + FT - 1) / FT, ty = (H + FT - 1) / FT;
   if (form == 0) {
-    hipLaunchKernelGGL(cl_fused, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, bound, sat, ms, acc,
-                       o_out);
+    hipLaunchKernelGGL(cl_fused, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, acc, o);
     return hipGetLastError();
   }
   // persistent: one work-group per CU, contiguous runs of tiles
@@ -1256,7 +1180,7 @@ hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, co
   const int ntiles = tx * ty * ndom;
   const int per = (ntiles + cus[dev] - 1) / cus[dev];
   const int groups = (ntiles + per - 1) / per;
-  hipLaunchKernelGGL(cl_fused_p, dim3(groups), dim3(512), 0, st, dom, par, wf, wl1, w, bound, sat, ms, acc, o_out,
-                     tx, ty, ntiles, per);
+  hipLaunchKernelGGL(cl_fused_p, dim3(groups), dim3(512), 0, st, dom, par, wf, wl1, w, acc, o, tx, ty, ntiles,
+                     per);
   return hipGetLastError();
 }

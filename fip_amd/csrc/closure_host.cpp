@@ -115,9 +115,11 @@ struct Attachment {
   ClMass* ms = nullptr;     // one per scenario
   double* acc = nullptr;    // 2 per scenario
   ClDom* dom = nullptr;     // one per scenario
+  float* o = nullptr;       // network outputs, [scenario][H][W][2]
   float sat = 2.65f;
   double limit = 1e-4;
   ~Attachment() {
+    if (o) hipFree(o);
     if (bstd) hipFree(bstd);
     if (zref) hipFree(zref);
     if (ms) hipFree(ms);
@@ -466,6 +468,7 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   const fip_grid& g = fip_internal::grid(coarse);
   const size_t spx = (size_t)g.W * g.H;
   int rc = dalloc(&a->bstd, spx);
+  if (!rc) rc = dalloc(&a->o, spx * 2);
   if (!rc) rc = dalloc(&a->ms, 1);
   if (!rc) rc = dalloc(&a->acc, 2);
   if (rc) return rc;
@@ -479,8 +482,8 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   cl->att.push_back(std::move(a));
   fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
     const fip_grid& g = fip_internal::grid(ap->sim);
-    CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, kBound, ap->sat, ap->ms, ap->acc,
-                         nullptr, st, cl->fused_form));
+    CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, ap->acc, ap->o, st, cl->fused_form));
+    CHIP(cl_correct_launch(g, par, ap->o, kBound, ap->sat, ap->ms, st));
     CHIP(cl_mass_check_launch(g, par, ap->acc, ap->ms, ap->limit, st));
     return FIP_OK;
   });
@@ -508,6 +511,7 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   a->limit = mass_limit;
   const size_t spx = (size_t)g0.W * g0.H;
   int rc = dalloc(&a->bstd, spx * n);
+  if (!rc) rc = dalloc(&a->o, spx * 2 * n);
   if (!rc) rc = dalloc(&a->zref, (size_t)n);
   if (!rc) rc = dalloc(&a->ms, (size_t)n);
   if (!rc) rc = dalloc(&a->acc, (size_t)2 * n);
@@ -525,9 +529,10 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   cl->att.push_back(std::move(a));
   fip_internal::ens_set_post_hook(ens, [cl, ap, n](hipStream_t st, int par) -> int {
     const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ap->ens, 0));
-    CHIP(cl_fused_launch(ap->dom, n, g0.W, g0.H, par, cl->wf, cl->wl1, cl->w, kBound, ap->sat, ap->ms, ap->acc,
-                         nullptr, st, cl->fused_form));
-    CHIP(cl_mass_check_ens_launch(fip_internal::ens_scn(ap->ens), n, par, ap->acc, ap->ms, ap->limit, st));
+    const fip_ens_scn* scn = fip_internal::ens_scn(ap->ens);
+    CHIP(cl_fused_launch(ap->dom, n, g0.W, g0.H, par, cl->wf, cl->wl1, cl->w, ap->acc, ap->o, st, cl->fused_form));
+    CHIP(cl_correct_ens_launch(scn, par, ap->o, kBound, ap->sat, ap->ms, g0.W, g0.H, n, st));
+    CHIP(cl_mass_check_ens_launch(scn, n, par, ap->acc, ap->ms, ap->limit, st));
     return FIP_OK;
   });
   return FIP_OK;
@@ -617,23 +622,18 @@ int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed
   const size_t spx = (size_t)g.W * g.H;
   float *bstd = nullptr, *o = nullptr;
   ClDom* dom = nullptr;
-  ClMass* ms = nullptr;
-  double* acc = nullptr;
   CHIP(hipMalloc(&bstd, spx * sizeof(float)));
   CHIP(hipMalloc(&o, spx * 2 * sizeof(float)));
   CHIP(hipMalloc(&dom, sizeof(ClDom)));
-  CHIP(hipMalloc(&ms, sizeof(ClMass)));
-  CHIP(hipMalloc(&acc, 2 * sizeof(double)));
-  CHIP(hipMemset(ms, 0, sizeof(ClMass)));
   CHIP(hipMemcpy(bstd, bed_std, spx * sizeof(float), hipMemcpyHostToDevice));
   ClDom hd{g, bstd, zref, 0};
   CHIP(hipMemcpy(dom, &hd, sizeof hd, hipMemcpyHostToDevice));
   // par such that par ^ 1 is the current side and the halt flag read is that of no step
   const int par = fip_internal::cur_side(coarse) ^ 1;
-  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, kBound, 1e30f, ms, acc, o, nullptr, cl->fused_form);
+  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, nullptr, o, nullptr, cl->fused_form);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(out, o, spx * 2 * sizeof(float), hipMemcpyDeviceToHost);
-  hipFree(bstd); hipFree(o); hipFree(dom); hipFree(ms); hipFree(acc);
+  hipFree(bstd); hipFree(o); hipFree(dom);
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("fused outputs: ") + hipGetErrorString(e));
   return FIP_OK;
 }

@@ -264,3 +264,30 @@ def test_persistent_fused_kernel_equals_tile_form(shape):
     assert st1["max_rel_mass_error"] < 1e-5, st1
     for x in sims + cls:
         x.close()
+
+
+def test_closure_ensemble_deterministic():
+    """Two runs of a closure-corrected ensemble give the same bits: the network reads the
+    uncorrected state of every tile's halo, and the correction is a separate launch."""
+    from fip_amd.sim import Closure, Ensemble
+    rng = np.random.default_rng(12)
+    p = rand_params(rng, 0.3)
+    runs = []
+    for _ in range(2):
+        cases, stds, zrefs = [], [], []
+        for sc in (21, 22, 23):
+            fine, coarse, bstd, zref = fine_coarse_case(320, 256, 1.0, 4, scenario=sc, wet=0.5)
+            cases.append(coarse)
+            stds.append(bstd)
+            zrefs.append(zref)
+        cl = Closure(80, 64, 0, seed=2)
+        cl.params = p
+        e = Ensemble(cases, mode="dense", variant="fast", tile_rows=32)
+        cl.attach_ensemble(e, np.stack(stds), np.array(zrefs, np.float32), sat=3.0)
+        e.set_run([1e9] * 3)
+        e.simulate(40)
+        runs.append([e.field(k, f) for k in range(3) for f in ("h", "qx", "qy")])
+        e.close()
+        cl.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
