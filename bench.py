@@ -287,9 +287,10 @@ def main():
         ev = res.pop("eval")
         rmse_ok = ev["learned_depth_rmse_m"] <= 0.02
         out["closure"] = {
-            "workload": (f"{args.closure_train} training + {args.closure_test} held-out scenarios, fine "
-                         f"1024^2 dx=1 m (ridged terrain, discharge inflow), coarse 256^2 dx=4 m, "
-                         f"{args.closure_duration:.0f} s simulated, fixed dt 0.05/0.2 s"),
+            "workload": (f"{args.closure_train} training scenarios (fine 1024^2 dx=1 m, coarse 256^2 dx=4 m) "
+                         f"+ {args.closure_test} held-out scenarios (fine 2048^2 dx=1 m, coarse 512^2 dx=4 m); "
+                         f"ridged terrain, discharge inflow, {args.closure_duration:.0f} s simulated, "
+                         f"fixed dt 0.05/0.2 s"),
             "speedup_learned_vs_fine": ev["speedup_learned_vs_fine"],
             "speedup_at_rmse_le_2cm": ev["speedup_learned_vs_fine"] if rmse_ok else None,
             "training": res, **{k: v for k, v in ev.items() if k not in ("scenarios", "duration_s")}}

@@ -183,9 +183,10 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
 
 
 def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1500, batch=32, patch=128,
-               lr=1e-3, W=1024, log=print):
-    """Generate samples, train, evaluate. The held-out samples come first in the store, the
-    training samples after them."""
+               lr=1e-3, W=1024, eval_W=2048, log=print):
+    """Generate samples (fine grids of W^2), train, evaluate on held-out scenarios with
+    fine grids of eval_W^2 (the network is convolutional, so it applies to any size). The
+    held-out samples come first in the store, the training samples after them."""
     from .sim import Closure
     Wc = W // F
     nsamp = n_train * int(duration // sample_every) + n_test * 4
@@ -203,12 +204,12 @@ def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1
            "heldout_flux_mse_plain": plain1, "heldout_flux_mse_learned": learned1,
            "heldout_flux_mse_untrained": learned0}
     log(str(res))
-    res["eval"] = evaluate(cl, range(1000, 1000 + n_test), duration, W=W, log=log)
+    res["eval"] = evaluate(cl, range(1000, 1000 + n_test), duration, W=eval_W, log=log)
     cl.close()
     return res
 
 
-def time_ensemble_step(steps=300, n=8, W=1024, warm=200, sat=1e30, seed=11, log=print):
+def time_ensemble_step(steps=300, n=8, W=1024, warm=200, sat=1e30, seed=11, form=None, log=print):
     """Per-step wall time of the held-out coarse twins (n scenarios of (W/4)^2) as a plain
     ensemble and with a closure of seeded random weights attached (every face corrected:
     sat = 1e30). Returns (plain_us, learned_us)."""
@@ -219,6 +220,8 @@ def time_ensemble_step(steps=300, n=8, W=1024, warm=200, sat=1e30, seed=11, log=
     bstd = np.stack([p[1] for p in pairs])
     zref = np.array([p[2] for p in pairs], np.float32)
     cl = Closure(W // F, W // F, max_samples=0, seed=seed)
+    if form is not None:
+        cl.set_kernel(form)
     out = []
     for label, closure in (("plain", None), ("learned", cl)):
         e = Ensemble(coarses, mode="dense", variant="fast", tile_rows=32)
@@ -244,5 +247,7 @@ if __name__ == "__main__":
     ap.add_argument("--scenarios", type=int, default=8)
     ap.add_argument("--fine", type=int, default=1024, help="fine grid size (the coarse one is a quarter)")
     ap.add_argument("--warm", type=int, default=200)
+    ap.add_argument("--form", type=int, default=None, help="fused kernel form (fip_closure_set_kernel)")
+    ap.add_argument("--sat", type=float, default=1e30, help="saturation limit of the network outputs")
     a = ap.parse_args()
-    time_ensemble_step(a.steps, a.scenarios, a.fine, a.warm)
+    time_ensemble_step(a.steps, a.scenarios, a.fine, a.warm, sat=a.sat, form=a.form)

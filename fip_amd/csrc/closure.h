@@ -50,8 +50,8 @@ struct ClDom {
 #define CL_PACKED_L1 (2 * 3 * 64 * 8)   // bf16 elements of the 8-channel packing of layer 0
 
 // Network outputs o ([ndom][H][W][2]) of every cell of ndom domains of one size, and their
-// mass-balance sums into acc (2 per domain) when acc != NULL. form 1: persistent kernel,
-// 0: one work-group per tile.
+// mass-balance sums into acc (2 per domain) when acc != NULL. form 2: row march,
+// 1: persistent tiles, 0: one work-group per tile.
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
                            const float* w, double* acc, float* o, hipStream_t st, int form = 1);
 hipError_t cl_pack_l1_launch(const float* w, u16* wl1, hipStream_t st);

@@ -438,11 +438,19 @@ __device__ __forceinline__ unsigned correct_cell(const fip_grid& g, int cur, con
   return nsat;
 }
 
-// one atomic per wavefront (every lane of the wavefront must call it)
-__device__ __forceinline__ void add_wave_count(unsigned* dst, unsigned n) {
+#define CR_ROWS 16   // rows per work-group of the correction kernels
+
+// one atomic per work-group of 256 threads (every thread must call it)
+__device__ __forceinline__ void add_block_count(unsigned* dst, unsigned n) {
+  __shared__ unsigned part[4];
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m);
-  if ((threadIdx.x & 63) == 0 && n) atomicAdd(dst, n);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(dst, t);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -452,15 +460,21 @@ __device__ __forceinline__ void add_wave_count(unsigned* dst, unsigned n) {
 // q += tanh(o) * bound * |q|, unless |o| exceeds `sat` (the network saturates: that face
 // keeps the plain step) or the domain's fallback flag is set. A corrected face moves
 // water between two cells of the domain, so the volume is conserved by construction.
-extern "C" __global__ void cl_correct(fip_grid g, int par, const float* o, float bound, float sat,
-                                      ClMass* ms) {
+// One work-group covers 256 columns x CR_ROWS rows, and adds its count of saturated faces
+// with one atomic: per-thread or per-wavefront atomics on the domain's single counter
+// serialise when many faces saturate.
+extern "C" __global__ __launch_bounds__(256) void cl_correct(fip_grid g, int par, const float* o, float bound,
+                                                            float sat, ClMass* ms) {
   if (g.ctl->halt_req[par ^ 1]) return;
   const int cur = par ^ 1;
   const bool off = ms->fallback != 0;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned nsat = 0;
-  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o, x, y, bound, sat, off);
-  add_wave_count(&ms->nsat, nsat);
+  for (int r = 0; r < CR_ROWS; r++) {
+    const int y = blockIdx.y * CR_ROWS + r;
+    if (x < g.W && y < g.H) nsat += correct_cell(g, cur, o, x, y, bound, sat, off);
+  }
+  add_block_count(&ms->nsat, nsat);
 }
 
 // Mass balance of a coarse domain without no_data cells, checked after each step before
@@ -526,17 +540,21 @@ extern "C" __global__ void cl_mass_check(fip_grid g, int par, double* acc, ClMas
 
 // Ensemble forms of cl_correct / cl_mass_check: blockIdx.z (or the thread
 // of cl_mass_check_ens) = scenario; o holds one W x H image per scenario.
-extern "C" __global__ void cl_correct_ens(const fip_ens_scn* scn, int par, const float* o, float bound,
-                                          float sat, ClMass* ms) {
+extern "C" __global__ __launch_bounds__(256) void cl_correct_ens(const fip_ens_scn* scn, int par, const float* o,
+                                                                float bound, float sat, ClMass* ms) {
   const int s = blockIdx.z;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
   const int cur = par ^ 1;
   const bool off = ms[s].fallback != 0;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const float* os = o + (long)s * g.W * g.H * 2;
   unsigned nsat = 0;
-  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o + (long)s * g.W * g.H * 2, x, y, bound, sat, off);
-  add_wave_count(&ms[s].nsat, nsat);
+  for (int r = 0; r < CR_ROWS; r++) {
+    const int y = blockIdx.y * CR_ROWS + r;
+    if (x < g.W && y < g.H) nsat += correct_cell(g, cur, os, x, y, bound, sat, off);
+  }
+  add_block_count(&ms[s].nsat, nsat);
 }
 
 extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms,
@@ -1040,6 +1058,234 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
 }
 
 // ---------------------------------------------------------------------------------------
+// Row-marching form of the fused inference step (form 2). A work-group takes a strip of
+// RS = 58 output columns and a span of rows [ya, yb) of one domain and marches along its
+// rows. Each layer keeps its last four rows in an LDS ring, so every row of every layer
+// is computed once: no vertical halo is recomputed, and the horizontal halo is 3 columns
+// per side at layer 0 (the tile forms recompute a 3-cell halo on all four sides). March
+// step k loads the features of row ya - 4 + k and layer L computes row ya - (6 + 2 L) + k,
+// so every input row was written in an earlier step; one barrier per step. 16 wavefronts: wavefront 4 L + j runs
+// layer L on pixel tile j (16 columns), and the four wavefronts on a SIMD are one of each
+// layer. Each keeps the A fragments of its layer in registers for the whole march. Every
+// output sums the same fragments in the same order as cl_fused: the same bits. The mass
+// terms come from the loaded discharges and depths (a 16-row ring); cl_correct(_ens)
+// applies the outputs.
+#define RS 58                  // output columns per strip
+#define RPX 65                 // pixel slots per ring row: 4 tiles of 16 + a spare slot
+#define RFW (RS + 8)           // feature columns (4 on each side)
+#define RRAW 16                // rows of the raw ring
+
+struct RmCtx {
+  const u16* fring;  // [4][RFW][8]
+  int x0, y0, W, H;  // strip origin, domain size
+};
+
+// matrix part of one layer row: NC K chunks, MT output blocks, for region column c
+// (clamped to the region) of row y; L1: features ring input
+template <int NC, int MT, bool L1>
+__device__ __forceinline__ void rm_mfma(const u16* in, int y, int c, const bf16x8 (&a)[NC][MT], f32x4 (&acc)[MT],
+                                        int grp) {
+#pragma unroll
+  for (int cc = 0; cc < NC; cc++) {
+    bf16x8 b;
+    if constexpr (L1) {
+      const int tap = 4 * cc + grp;
+      const int dy = tap < CL_TAPS ? tap / 3 - 1 : 0, dx = tap < CL_TAPS ? tap % 3 - 1 : 0;
+      const bf16x8 v = ld8(in + (((y + dy) & 3) * RFW + c + 1 + dx) * 8);
+      b = tap < CL_TAPS ? v : zero8();
+    } else {
+      const int dy = cc / 3 - 1, dx = cc % 3 - 1;
+      b = ld8(in + (((y + dy) & 3) * RPX + c + 1 + dx) * FPS + 8 * grp);
+    }
+#pragma unroll
+    for (int m = 0; m < MT; m++) acc[m] = mfma(a[cc][m], b, acc[m]);
+  }
+}
+
+// GELU epilogue of one layer row into its ring: region column c of a region of width RW
+// whose column 0 sits at x = x0 + c0off; lanes past the region store to the spare slot
+template <int RW>
+__device__ __forceinline__ void rm_store(u16* out, int y, int c, int x, int W, int H, const f32x4 (&acc)[2],
+                                         int grp) {
+  const bool valid = c < RW;
+  const bool inside = x >= 0 && x < W && y >= 0 && y < H;
+  const uint32_t keep = 0u - (uint32_t)inside;
+  const int px = valid ? c : RPX - 1;
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    const int co0 = m * 16 + 4 * grp;
+    const uint32_t lo = pack_bf2(gelu(acc[m][0]), gelu(acc[m][1])) & keep;
+    const uint32_t hi = pack_bf2(gelu(acc[m][2]), gelu(acc[m][3])) & keep;
+    *(uint2*)(out + ((y & 3) * RPX + px) * FPS + co0) = make_uint2(lo, hi);
+  }
+}
+
+__device__ __forceinline__ void rm_load_a(const u16* wp, int NC, int MT, int lane, bf16x8* a) {
+  for (int cc = 0; cc < NC; cc++)
+    for (int m = 0; m < MT; m++) a[cc * MT + m] = ld8(wp + ((m * NC + cc) * 64 + lane) * 8);
+}
+
+#define RNW 16                 // wavefronts per work-group of cl_fused_r: 4 per layer
+
+// One wavefront of layer L (0..3) on pixel tile j over the march steps: reads its input
+// ring `in`, writes its output ring `out` (layer 3: the outputs o and the mass terms; it
+// also loads the features of the next row and writes them to `out`, the feature ring).
+// Layer L computes row ya - (6 + 2 L) + k at step k.
+template <int L>
+__device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u16* in, u16* out, int j, int lane,
+                                         int x0, int ya, int nrows, int W, int H, float* od, float* raw,
+                                         double& v, double& fl, bool mass, const ClDom& D, int cur) {
+  constexpr int NC = L == 0 ? 3 : CL_TAPS, MT = L == 3 ? 1 : 2;
+  constexpr int RW = RS + 6 - 2 * L;       // region width
+  constexpr int M = 6 + 2 * L;             // row of step k: ya - M + k
+  constexpr int HALO = 3 - L;              // rows the next layer needs: [ya - HALO, yb + HALO)
+  const int col = lane & 15, grp = lane >> 4;
+  const int c = 16 * j + col;
+  const int cc = c < RW ? c : RW - 1;      // clamped for the loads
+  bf16x8 a[NC][MT];
+  rm_load_a(wp, NC, MT, lane, &a[0][0]);
+  f32x4 bias[MT];
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int co0 = m * 16 + 4 * grp;
+    bias[m] = (f32x4){bp[co0], bp[co0 + 1], bp[co0 + 2], bp[co0 + 3]};
+  }
+  // layer 3 also loads the features: thread fc of the feature region, x = x0 + fc - 4
+  const int fc = (int)threadIdx.x - 3 * 4 * 64;
+  const bool loader = L == 3 && fc >= 0 && fc < RFW;
+  const int fx = x0 + fc - 4;
+  float rb = 0.f, rh = 0.f, rqx = 0.f, rqy = 0.f, rbs = 0.f, rk = 0.f;
+  const auto load_row = [&](int yy) {
+    rb = rh = rqx = rqy = rbs = rk = 0.f;
+    if (loader && fx >= 0 && fx < W && yy >= 0 && yy < H) {
+      const long o = (long)yy * D.g.P + fx;
+      rb = D.g.bed[o];
+      rh = D.g.h[cur][o];
+      rqx = D.g.qx[cur][o];
+      rqy = D.g.qy[cur][o];
+      rbs = D.bstd[(long)yy * W + fx];
+      rk = D.g.kst ? D.g.kst[o] : D.g.kst_uniform;
+    }
+  };
+  if (L == 3) load_row(ya - 4);
+  const int K = nrows + 12;
+  for (int k = 0; k < K; k++) {
+    if constexpr (L == 3) {
+      const int yF = ya - 4 + k;
+      if (k < nrows + 8 && loader) {  // features of row yF (the formulas of feat_pixel)
+        uint4 pk = make_uint4(0, 0, 0, 0);
+        if (fx >= 0 && fx < W && yF >= 0 && yF < H) {
+          const float f0v = (rh + rb - D.zref) * 0.25f;
+          const float f3v = (rb - D.zref) * 0.25f;
+          const float f4v = rbs * 2.0f;
+          const float f5v = rk * (1.0f / 30.0f);
+          pk.x = (uint32_t)f2bf(f0v) | ((uint32_t)f2bf(rqx) << 16);
+          pk.y = (uint32_t)f2bf(rqy) | ((uint32_t)f2bf(f3v) << 16);
+          pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
+        }
+        *(uint4*)(out + ((yF & 3) * RFW + fc) * 8) = pk;  // layer 3's out: the feature ring
+        if (fc >= 4 && fc < RS + 4) {
+          float* q = raw + ((yF & (RRAW - 1)) * RS + fc - 4) * 3;
+          q[0] = rqx;
+          q[1] = rqy;
+          q[2] = rh;
+        }
+      }
+      if (k + 1 < nrows + 8) load_row(yF + 1);  // lands during this step
+    }
+    const int y = ya - M + k;
+    if (k >= M - HALO && k < nrows + M + HALO) {
+      f32x4 acc[MT];
+#pragma unroll
+      for (int m = 0; m < MT; m++) acc[m] = bias[m];
+      if constexpr (L == 3)
+        rm_mfma<NC, MT, false>(in, y, cc, a, acc, grp);
+      else
+        rm_mfma<NC, MT, L == 0>(in, y, cc, a, acc, grp);
+      if constexpr (L < 3) {
+        rm_store<RW>(out, y, c, x0 + c - HALO, W, H, acc, grp);
+      } else {  // outputs 0 and 1 (lane group 0) of row y, and its mass terms
+        const int x = x0 + c;
+        if (grp == 0 && c < RS && x < W) {
+          od[((long)y * W + x) * 2] = acc[0][0];
+          od[((long)y * W + x) * 2 + 1] = acc[0][1];
+          if (mass) {
+            const float* q = raw + ((y & (RRAW - 1)) * RS + c) * 3;
+            const bool rows = y >= 2 && y <= H - 3, cols = x >= 2 && x <= W - 3;
+            if (rows && cols) v += (double)q[2];
+            if (rows && x == W - 3) fl += (double)q[0];
+            if (rows && x == 1) fl -= (double)q[0];
+            if (cols && y == H - 3) fl += (double)q[1];
+            if (cols && y == 1) fl -= (double)q[1];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+extern "C" __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __restrict__ dom, int par,
+                                                             const u16* wf, const u16* wl1, const float* w,
+                                                             double* acc_out, float* o_out, int nstrip, int nspan,
+                                                             int span) {
+  __shared__ __attribute__((aligned(16))) u16 fring[4 * RFW * 8];
+  __shared__ __attribute__((aligned(16))) u16 ring1[4 * RPX * FPS];
+  __shared__ __attribute__((aligned(16))) u16 ring2[4 * RPX * FPS];
+  __shared__ __attribute__((aligned(16))) u16 ring3[4 * RPX * FPS];
+  __shared__ float raw[RRAW * RS * 3];
+  __shared__ double red[RNW * 2];
+  const int item = blockIdx.x;
+  const int strip = item % nstrip, rest = item / nstrip;
+  const int p = rest % nspan, d = rest / nspan;
+  const ClDom& D = dom[d];
+  if (D.g.ctl->halt_req[par ^ 1]) return;  // the whole work-group
+  const int cur = par ^ 1;
+  const int W = D.g.W, H = D.g.H;
+  const int x0 = strip * RS, ya = p * span;
+  const int yb = ya + span < H ? ya + span : H;
+  const int nrows = yb - ya;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int layer = wave >> 2, j = wave & 3;
+  float* od = o_out + (long)d * W * H * 2;
+  const bool mass = acc_out != nullptr;
+  double v = 0.0, fl = 0.0;
+  const float* bias = w + CL_TAPS * CL_C * CL_C;  // + layer * CL_LAYER_PARAMS
+  if (layer == 0)
+    rm_march<0>(wl1, bias, fring, ring1, j, lane, x0, ya, nrows, W, H, od, raw, v, fl, mass, D, cur);
+  else if (layer == 1)
+    rm_march<1>(wf + CL_PACKED, bias + CL_LAYER_PARAMS, ring1, ring2, j, lane, x0, ya, nrows, W, H, od, raw, v,
+                fl, mass, D, cur);
+  else if (layer == 2)
+    rm_march<2>(wf + 2 * CL_PACKED, bias + 2 * CL_LAYER_PARAMS, ring2, ring3, j, lane, x0, ya, nrows, W, H, od,
+                raw, v, fl, mass, D, cur);
+  else
+    rm_march<3>(wf + 3 * CL_PACKED, bias + 3 * CL_LAYER_PARAMS, ring3, fring, j, lane, x0, ya, nrows, W, H, od,
+                raw, v, fl, mass, D, cur);
+  if (mass) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      v += __shfl_xor(v, m);
+      fl += __shfl_xor(fl, m);
+    }
+    if (lane == 0) {
+      red[2 * wave] = v;
+      red[2 * wave + 1] = fl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sv = 0.0, sf = 0.0;
+      for (int k = 0; k < RNW; k++) {
+        sv += red[2 * k];
+        sf += red[2 * k + 1];
+      }
+      atomicAdd(&acc_out[2 * d], sv);
+      atomicAdd(&acc_out[2 * d + 1], sf);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Targets of a training sample: the fine domain's current face discharges restricted as
 // in cl_restrict, [pixel][2] fp32, and the mask of trainable faces (the ones cl_correct
 // corrects).
@@ -1148,12 +1394,12 @@ CL_WRAP(cl_adam, (float* w, float* m, float* v, const float* g, int nlayers, flo
                   float eps, float c1, float c2, hipStream_t st),
         P(w, m, v, g, nlayers, lr, b1, b2, eps, c1, c2), dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))
 CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms, hipStream_t st),
-        P(g, par, o, bound, sat, ms), dim3((g.W + 255) / 256, g.H), dim3(256))
+        P(g, par, o, bound, sat, ms), dim3((g.W + 255) / 256, (g.H + CR_ROWS - 1) / CR_ROWS), dim3(256))
 CL_WRAP(cl_mass_check, (fip_grid g, int par, double* acc, ClMass* ms, double limit, hipStream_t st),
         P(g, par, acc, ms, limit), dim3(1), dim3(64))
 CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, int par, const float* o, float bound, float sat, ClMass* ms, int W,
                          int H, int n, hipStream_t st),
-        P(scn, par, o, bound, sat, ms), dim3((W + 255) / 256, H, n), dim3(256))
+        P(scn, par, o, bound, sat, ms), dim3((W + 255) / 256, (H + CR_ROWS - 1) / CR_ROWS, n), dim3(256))
 CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
                             hipStream_t st),
         P(scn, n, par, acc, ms, limit), dim3((n + 63) / 64), dim3(64))
@@ -1167,7 +1413,6 @@ hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, co
     hipLaunchKernelGGL(cl_fused, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, acc, o);
     return hipGetLastError();
   }
-  // persistent: one work-group per CU, contiguous runs of tiles
   static int cus[64] = {};
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -1177,6 +1422,28 @@ hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, co
     e = hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
   }
+  if (form == 2) {
+    // row march: strips of RS columns, spans of rows chosen so that the work-groups (one
+    // per CU at a time) finish in the fewest march steps, counting 12 warm-up steps each
+    const int nstrip = (W + RS - 1) / RS;
+    int best = H, best_cost = -1;
+    for (int span = 16; span < H + 16; span += 16) {
+      const int sp = span < H ? span : H;
+      const long items = (long)ndom * nstrip * ((H + sp - 1) / sp);
+      const long rounds = (items + cus[dev] - 1) / cus[dev];
+      const long cost = rounds * (sp + 12);
+      if (best_cost < 0 || cost < best_cost) {
+        best_cost = cost;
+        best = sp;
+      }
+      if (sp == H) break;
+    }
+    const int nspan = (H + best - 1) / best;
+    hipLaunchKernelGGL(cl_fused_r, dim3(ndom * nstrip * nspan), dim3(64 * RNW), 0, st, dom, par, wf, wl1, w, acc,
+                       o, nstrip, nspan, best);
+    return hipGetLastError();
+  }
+  // persistent: one work-group per CU, contiguous runs of tiles
   const int ntiles = tx * ty * ndom;
   const int per = (ntiles + cus[dev] - 1) / cus[dev];
   const int groups = (ntiles + per - 1) / per;

@@ -142,7 +142,7 @@ struct fip_closure {
   u16* wl1 = nullptr;   // layer 0 in the 8-channel packing of the fused inference kernel
   float* part = nullptr;
   int t = 0;            // Adam steps taken
-  int fused_form = 1;   // launch form of the fused inference kernel (cl_fused_launch)
+  int fused_form = 2;   // launch form of the fused inference kernel (cl_fused_launch)
   // training samples
   int Wc = 0, Hc = 0, cap = 0, ns = 0;
   u16* s_feat = nullptr;     // [cap][Hc][Wc][8]
@@ -640,7 +640,8 @@ int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed
 
 int fip_closure_set_kernel(fip_closure* cl, int form) {
   if (!cl) return fail(FIP_E_INVALID, "null argument");
-  if (form != 0 && form != 1) return fail(FIP_E_INVALID, "kernel form must be 0 (per tile) or 1 (persistent)");
+  if (form < 0 || form > 2)
+    return fail(FIP_E_INVALID, "kernel form must be 0 (per tile), 1 (persistent tiles) or 2 (row march)");
   cl->fused_form = form;
   return FIP_OK;
 }

@@ -232,21 +232,22 @@ def test_fused_step_matches_layerwise_forward():
     cl.close()
 
 
-@pytest.mark.parametrize("shape", [(96, 64), (50, 37)])
-def test_persistent_fused_kernel_equals_tile_form(shape):
-    """The persistent fused kernel (weights in LDS, prefetched tiles) gives the tile
-    form's outputs and corrections bit for bit, on a multiple and a non-multiple of the
-    16 x 16 tile."""
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("shape", [(96, 64), (50, 37), (130, 75)])
+def test_fused_kernel_forms_agree(shape, form):
+    """The persistent-tile form (weights in LDS, prefetched tiles) and the row-march form
+    (58-column strips, LDS row rings) give the tile form's outputs and corrections bit for
+    bit, on a multiple and non-multiples of the 16 x 16 tile and of the 58-column strip."""
     from fip_amd.sim import Closure, Sim
     Wc, Hc = shape
     fine, coarse, bstd, zref = fine_coarse_case(Wc * 4, Hc * 4, 1.0, 4, scenario=7, wet=0.5)
     rng = np.random.default_rng(5)
     p = rand_params(rng, 0.3)
     cls, sims, outs = [], [], []
-    for form in (0, 1):
+    for f in (0, form):
         cl = Closure(Wc, Hc, 0, seed=2)
         cl.params = p
-        cl.set_kernel(form)
+        cl.set_kernel(f)
         s = Sim(coarse, mode="dense")
         s.set_run(1e9)
         s.simulate(20)
