@@ -78,10 +78,13 @@ hipError_t cl_loss_launch(const float* o, const float* qp, const float* qt, cons
 hipError_t cl_pack_launch(const float* w, int nlayers, u16* wf, u16* wb, hipStream_t st);
 hipError_t cl_adam_launch(float* w, float* m, float* v, const float* g, int nlayers, float lr, float b1,
                           float b2, float eps, float c1, float c2, hipStream_t st);
+// Correction of the current face discharges by o; saturated faces counted per work-group
+// of 256 columns of a row into part ((W + 255) / 256 * H slots per domain).
 hipError_t cl_correct_launch(fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms,
-                             hipStream_t st);
-hipError_t cl_mass_check_launch(fip_grid g, int par, double* acc, ClMass* ms, double limit, hipStream_t st);
+                             unsigned* part, hipStream_t st);
+hipError_t cl_mass_check_launch(fip_grid g, int par, double* acc, ClMass* ms, double limit, const unsigned* part,
+                                int nparts, hipStream_t st);
 hipError_t cl_correct_ens_launch(const fip_ens_scn* scn, int par, const float* o, float bound, float sat,
-                                 ClMass* ms, int W, int H, int n, hipStream_t st);
+                                 ClMass* ms, unsigned* part, int W, int H, int n, hipStream_t st);
 hipError_t cl_mass_check_ens_launch(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
-                                    hipStream_t st);
+                                    const unsigned* part, int nparts, hipStream_t st);

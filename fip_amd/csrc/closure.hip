@@ -438,19 +438,25 @@ __device__ __forceinline__ unsigned correct_cell(const fip_grid& g, int cur, con
   return nsat;
 }
 
-#define CR_ROWS 16   // rows per work-group of the correction kernels
-
-// one atomic per work-group of 256 threads (every thread must call it)
-__device__ __forceinline__ void add_block_count(unsigned* dst, unsigned n) {
-  __shared__ unsigned part[4];
+// The saturated faces of a work-group of 256 threads, written to its slot of `part`
+// (every thread must call it). The mass check adds the slots of a domain: one atomic per
+// work-group on the domain's single counter would serialise.
+__device__ __forceinline__ void block_count(unsigned* part, unsigned n) {
+  __shared__ unsigned wsum[4];
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = part[0] + part[1] + part[2] + part[3];
-    if (t) atomicAdd(dst, t);
-  }
+  if (threadIdx.x == 0) *part = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// sum of nparts counts by one wavefront (lane 0 holds the result)
+__device__ __forceinline__ unsigned wave_sum_parts(const unsigned* part, int nparts) {
+  unsigned t = 0;
+  for (int i = threadIdx.x & 63; i < nparts; i += 64) t += part[i];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) t += __shfl_xor(t, m);
+  return t;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -460,21 +466,17 @@ __device__ __forceinline__ void add_block_count(unsigned* dst, unsigned n) {
 // q += tanh(o) * bound * |q|, unless |o| exceeds `sat` (the network saturates: that face
 // keeps the plain step) or the domain's fallback flag is set. A corrected face moves
 // water between two cells of the domain, so the volume is conserved by construction.
-// One work-group covers 256 columns x CR_ROWS rows, and adds its count of saturated faces
-// with one atomic: per-thread or per-wavefront atomics on the domain's single counter
-// serialise when many faces saturate.
+// One work-group per 256 columns of a row; its count of saturated faces goes to
+// part[blockIdx.y * gridDim.x + blockIdx.x].
 extern "C" __global__ __launch_bounds__(256) void cl_correct(fip_grid g, int par, const float* o, float bound,
-                                                            float sat, ClMass* ms) {
+                                                            float sat, ClMass* ms, unsigned* part) {
   if (g.ctl->halt_req[par ^ 1]) return;
   const int cur = par ^ 1;
   const bool off = ms->fallback != 0;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   unsigned nsat = 0;
-  for (int r = 0; r < CR_ROWS; r++) {
-    const int y = blockIdx.y * CR_ROWS + r;
-    if (x < g.W && y < g.H) nsat += correct_cell(g, cur, o, x, y, bound, sat, off);
-  }
-  add_block_count(&ms->nsat, nsat);
+  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o, x, y, bound, sat, off);
+  block_count(part + (long)blockIdx.y * gridDim.x + blockIdx.x, nsat);
 }
 
 // Mass balance of a coarse domain without no_data cells, checked after each step before
@@ -532,37 +534,42 @@ __device__ __forceinline__ void mass_check(const fip_grid& g, double* acc, ClMas
   acc[1] = 0.0;
 }
 
-extern "C" __global__ void cl_mass_check(fip_grid g, int par, double* acc, ClMass* ms, double limit) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wavefront: adds the saturation counts of the step (nparts slots), then the check.
+extern "C" __global__ void cl_mass_check(fip_grid g, int par, double* acc, ClMass* ms, double limit,
+                                         const unsigned* part, int nparts) {
   if (g.ctl->halt_req[par ^ 1]) return;
+  const unsigned t = wave_sum_parts(part, nparts);
+  if (threadIdx.x != 0) return;
+  ms->nsat += t;
   mass_check(g, acc, *ms, limit);
 }
 
-// Ensemble forms of cl_correct / cl_mass_check: blockIdx.z (or the thread
-// of cl_mass_check_ens) = scenario; o holds one W x H image per scenario.
+// Ensemble forms of cl_correct / cl_mass_check: blockIdx.z (cl_mass_check_ens:
+// blockIdx.x) = scenario; o holds one W x H image per scenario, part nparts slots per
+// scenario.
 extern "C" __global__ __launch_bounds__(256) void cl_correct_ens(const fip_ens_scn* scn, int par, const float* o,
-                                                                float bound, float sat, ClMass* ms) {
+                                                                float bound, float sat, ClMass* ms,
+                                                                unsigned* part) {
   const int s = blockIdx.z;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
   const int cur = par ^ 1;
   const bool off = ms[s].fallback != 0;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const float* os = o + (long)s * g.W * g.H * 2;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   unsigned nsat = 0;
-  for (int r = 0; r < CR_ROWS; r++) {
-    const int y = blockIdx.y * CR_ROWS + r;
-    if (x < g.W && y < g.H) nsat += correct_cell(g, cur, os, x, y, bound, sat, off);
-  }
-  add_block_count(&ms[s].nsat, nsat);
+  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o + (long)s * g.W * g.H * 2, x, y, bound, sat, off);
+  const long nparts = (long)gridDim.x * gridDim.y;
+  block_count(part + s * nparts + (long)blockIdx.y * gridDim.x + blockIdx.x, nsat);
 }
 
 extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms,
-                                             double limit) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+                                             double limit, const unsigned* part, int nparts) {
+  const int s = blockIdx.x;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
+  const unsigned t = wave_sum_parts(part + (long)s * nparts, nparts);
+  if (threadIdx.x != 0) return;
+  ms[s].nsat += t;
   mass_check(g, acc + 2 * s, ms[s], limit);
 }
 
@@ -1393,16 +1400,18 @@ CL_WRAP(cl_pack, (const float* w, int nlayers, u16* wf, u16* wb, hipStream_t st)
 CL_WRAP(cl_adam, (float* w, float* m, float* v, const float* g, int nlayers, float lr, float b1, float b2,
                   float eps, float c1, float c2, hipStream_t st),
         P(w, m, v, g, nlayers, lr, b1, b2, eps, c1, c2), dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))
-CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms, hipStream_t st),
-        P(g, par, o, bound, sat, ms), dim3((g.W + 255) / 256, (g.H + CR_ROWS - 1) / CR_ROWS), dim3(256))
-CL_WRAP(cl_mass_check, (fip_grid g, int par, double* acc, ClMass* ms, double limit, hipStream_t st),
-        P(g, par, acc, ms, limit), dim3(1), dim3(64))
-CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, int par, const float* o, float bound, float sat, ClMass* ms, int W,
-                         int H, int n, hipStream_t st),
-        P(scn, par, o, bound, sat, ms), dim3((W + 255) / 256, (H + CR_ROWS - 1) / CR_ROWS, n), dim3(256))
+CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms, unsigned* part,
+                     hipStream_t st),
+        P(g, par, o, bound, sat, ms, part), dim3((g.W + 255) / 256, g.H), dim3(256))
+CL_WRAP(cl_mass_check, (fip_grid g, int par, double* acc, ClMass* ms, double limit, const unsigned* part,
+                        int nparts, hipStream_t st),
+        P(g, par, acc, ms, limit, part, nparts), dim3(1), dim3(64))
+CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, int par, const float* o, float bound, float sat, ClMass* ms,
+                         unsigned* part, int W, int H, int n, hipStream_t st),
+        P(scn, par, o, bound, sat, ms, part), dim3((W + 255) / 256, H, n), dim3(256))
 CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
-                            hipStream_t st),
-        P(scn, n, par, acc, ms, limit), dim3((n + 63) / 64), dim3(64))
+                            const unsigned* part, int nparts, hipStream_t st),
+        P(scn, n, par, acc, ms, limit, part, nparts), dim3(n), dim3(64))
 CL_WRAP(cl_pack_l1, (const float* w, u16* wl1, hipStream_t st), P(w, wl1), dim3((CL_TAPS * 8 * CL_C + 255) / 256),
         dim3(256))
 

@@ -116,10 +116,13 @@ struct Attachment {
   double* acc = nullptr;    // 2 per scenario
   ClDom* dom = nullptr;     // one per scenario
   float* o = nullptr;       // network outputs, [scenario][H][W][2]
+  unsigned* part = nullptr; // saturated faces per correction work-group, [scenario][nparts]
+  int nparts = 0;
   float sat = 2.65f;
   double limit = 1e-4;
   ~Attachment() {
     if (o) hipFree(o);
+    if (part) hipFree(part);
     if (bstd) hipFree(bstd);
     if (zref) hipFree(zref);
     if (ms) hipFree(ms);
@@ -467,8 +470,10 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   a->limit = mass_limit;
   const fip_grid& g = fip_internal::grid(coarse);
   const size_t spx = (size_t)g.W * g.H;
+  a->nparts = (g.W + 255) / 256 * g.H;
   int rc = dalloc(&a->bstd, spx);
   if (!rc) rc = dalloc(&a->o, spx * 2);
+  if (!rc) rc = dalloc(&a->part, (size_t)a->nparts);
   if (!rc) rc = dalloc(&a->ms, 1);
   if (!rc) rc = dalloc(&a->acc, 2);
   if (rc) return rc;
@@ -483,8 +488,8 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
     const fip_grid& g = fip_internal::grid(ap->sim);
     CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, ap->acc, ap->o, st, cl->fused_form));
-    CHIP(cl_correct_launch(g, par, ap->o, kBound, ap->sat, ap->ms, st));
-    CHIP(cl_mass_check_launch(g, par, ap->acc, ap->ms, ap->limit, st));
+    CHIP(cl_correct_launch(g, par, ap->o, kBound, ap->sat, ap->ms, ap->part, st));
+    CHIP(cl_mass_check_launch(g, par, ap->acc, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
   });
   return FIP_OK;
@@ -510,8 +515,10 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   a->sat = sat;
   a->limit = mass_limit;
   const size_t spx = (size_t)g0.W * g0.H;
+  a->nparts = (g0.W + 255) / 256 * g0.H;
   int rc = dalloc(&a->bstd, spx * n);
   if (!rc) rc = dalloc(&a->o, spx * 2 * n);
+  if (!rc) rc = dalloc(&a->part, (size_t)a->nparts * n);
   if (!rc) rc = dalloc(&a->zref, (size_t)n);
   if (!rc) rc = dalloc(&a->ms, (size_t)n);
   if (!rc) rc = dalloc(&a->acc, (size_t)2 * n);
@@ -531,8 +538,8 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
     const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ap->ens, 0));
     const fip_ens_scn* scn = fip_internal::ens_scn(ap->ens);
     CHIP(cl_fused_launch(ap->dom, n, g0.W, g0.H, par, cl->wf, cl->wl1, cl->w, ap->acc, ap->o, st, cl->fused_form));
-    CHIP(cl_correct_ens_launch(scn, par, ap->o, kBound, ap->sat, ap->ms, g0.W, g0.H, n, st));
-    CHIP(cl_mass_check_ens_launch(scn, n, par, ap->acc, ap->ms, ap->limit, st));
+    CHIP(cl_correct_ens_launch(scn, par, ap->o, kBound, ap->sat, ap->ms, ap->part, g0.W, g0.H, n, st));
+    CHIP(cl_mass_check_ens_launch(scn, n, par, ap->acc, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
   });
   return FIP_OK;
