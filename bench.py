@@ -111,15 +111,20 @@ def run_ensemble(variant, steps, warmup, nscn=64):
     return out
 
 
-def full_run(variant, rows, chunk=5000, max_steps=0):
+def full_run(variant, chunk=5000, max_steps=0):
     """The 5667 x 6500 workload over its whole 12 h from the dry start in masked mode:
     steps, wall time, effective rate (W*H per step) and the active-tile fraction sampled
-    every `chunk` steps. Returns a dict for a JSON line."""
-    from fip_amd.sim import Sim
+    every `chunk` steps. Tile rows and launch shape come from the autotuner run on the
+    30 %-wet state (the late part of the run, where the time goes). Returns a dict for a
+    JSON line."""
+    from fip_amd.sim import Sim, tune
     from fip_amd.workloads import config1
+    best = tune(wet_state(config1()), mode="masked", variant=variant, steps=10)
+    rows = best["tile_rows"]
     case = config1()
     valid = case.bed != np.float32(case.no_data)
     s = Sim(case, mode="masked", variant=variant, tile_rows=rows)
+    s.set_launch(best["waves_per_block"], best["blocks_per_cu"])
     s.set_run(case.duration)
     samples = []
     steps = 0
@@ -140,13 +145,55 @@ def full_run(variant, rows, chunk=5000, max_steps=0):
     vol = float(h.astype(np.float64)[valid].sum() * case.dx * case.dx)
     out = {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q hydrograph "
                        "0->600->0 m3/s over 12 h, right open, dry start, masked mode",
-           "variant": variant, "tile_rows": rows, "simulated_s": round(s.time()[0], 2), "steps": steps,
+           "variant": variant, "tuned": {k: best[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")},
+           "simulated_s": round(s.time()[0], 2), "steps": steps,
            "wall_s": round(wall, 2), "ms_per_step": round(wall / steps * 1e3, 4),
            "gcells_effective": round(case.W * case.H * steps / wall / 1e9, 2),
            "simulated_h_per_wall_s": round(s.time()[0] / 3600.0 / wall, 3),
            "final_wet_fraction_of_valid": round(wet, 4), "final_volume_m3": round(vol, 1),
            "active_tile_fraction": samples}
     s.close()
+    return out
+
+
+def ensemble_full_run(variant, nscn=64, chunk=2000):
+    """The 64-scenario batched ensemble as specified: every scenario from its dry start
+    over its whole 6 h, masked mode, until every scenario has finished. Tile rows and
+    launch shape come from the ensemble autotuner run on the scenarios 30 % wet (tuned on
+    the dry start, where few tiles are active, it picks shapes that are slow later).
+    Returns a dict for a JSON line."""
+    from fip_amd.sim import Ensemble, tune_ensemble
+    from fip_amd.workloads import config2
+    cases = [config2(s) for s in range(nscn)]
+    cells = sum(c.W * c.H for c in cases)
+    best = tune_ensemble([wet_state(config2(s)) for s in range(nscn)], mode="masked", variant=variant, steps=10)
+    e = Ensemble(cases, mode="masked", variant=variant, tile_rows=best["tile_rows"])
+    e.set_launch(best["waves_per_block"], best["blocks_per_cu"])
+    e.set_run([c.duration for c in cases])
+    t0 = time.perf_counter()
+    steps = 0
+    cell_steps = 0
+    while True:
+        before = [e.time(k)[2] for k in range(nscn)]
+        act, done = e.simulate(chunk)
+        after = [e.time(k)[2] for k in range(nscn)]
+        steps += done
+        cell_steps += sum((a - b) * c.W * c.H for a, b, c in zip(after, before, cases))
+        print(f"ensemble full run: {steps} steps, {act} scenarios active", file=sys.stderr, flush=True)
+        if act == 0 or done == 0:
+            break
+    wall = time.perf_counter() - t0
+    st = e.stats()
+    sim_h = sum(e.time(k)[0] for k in range(nscn)) / 3600.0
+    iters = [e.time(k)[2] for k in range(nscn)]
+    out = {"workload": f"{nscn} scenarios (sizes 256^2 .. 2048x1536, {cells} cells), dry start, q "
+                       f"hydrographs, 6 h each, masked mode, hipGraph step loop",
+           "variant": variant, "tuned": {k: best[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")},
+           "ensemble_steps": steps, "scenario_steps_min_max": [min(iters), max(iters)],
+           "wall_s": round(wall, 2), "gcells_effective": round(cell_steps / wall / 1e9, 2),
+           "simulated_h_per_wall_s": round(sim_h / wall, 2),
+           "tiles_active_last_step": [st["tiles_computed"], st["tiles_total"]]}
+    e.close()
     return out
 
 
@@ -211,6 +258,9 @@ def main():
     ap.add_argument("--full-run", action="store_true",
                     help="run the 5667x6500 workload over its whole 12 h from the dry start (masked) "
                          "and print that leg's JSON line only")
+    ap.add_argument("--ensemble-full-run", action="store_true",
+                    help="run the 64-scenario ensemble from its dry start over its whole 6 h (masked) "
+                         "and print that leg's JSON line only")
     ap.add_argument("--full-run-steps", type=int, default=0,
                     help="stop the full run after this many steps (0: the whole 12 h)")
     ap.add_argument("--calibrate", action="store_true",
@@ -220,7 +270,10 @@ def main():
         calibrate_copy()
         return
     if args.full_run:
-        print(json.dumps(full_run(args.variant, args.rows, max_steps=args.full_run_steps)), flush=True)
+        print(json.dumps(full_run(args.variant, max_steps=args.full_run_steps)), flush=True)
+        return
+    if args.ensemble_full_run:
+        print(json.dumps(ensemble_full_run(args.variant)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the single-domain step does not shard "

@@ -40,7 +40,7 @@ typedef struct fip_ctl {
   int pre_sampled;       // the pre-first-step kernel logged the sample of the first call
   // active-tile schedule (masked mode), double-buffered by iteration parity
   unsigned list_count[2];
-  unsigned work_next[2];
+  unsigned work_next[2];   // reset each step; the list kernels stride and do not use it
   // boundary-condition parameters of the last BC application
   float bc_q[FIP_MAX_BC];
   float bc_z[FIP_MAX_BC];
@@ -108,6 +108,6 @@ typedef struct fip_ens_chunk {
 // (scenario << 40) | (tile << 12) | (lo << 6) | hi.
 typedef struct fip_ens_ctl {
   unsigned list_count[2];
-  unsigned work_next[2];
+  unsigned work_next[2];   // reset each step; the list kernels stride and do not use it
   int finished[FIP_ENS_MAX];  // set by the post kernel when a scenario reached its target
 } fip_ens_ctl;
