@@ -46,9 +46,11 @@ def wet_state(case, frac=0.30):
     return case
 
 
-def run_rate(case, mode, variant, rows, steps, warmup, timed_steps):
+def run_rate(case, mode, variant, rows, steps, warmup, timed_steps, launch=None):
     from fip_amd.sim import Sim
     s = Sim(case, mode=mode, variant=variant, tile_rows=rows)
+    if launch is not None:
+        s.set_launch(*launch)
     s.set_run(1e30)
     s.simulate(warmup)
     s.sync()
@@ -324,12 +326,15 @@ def main():
         out[f"dense_{other}"] = {"gcells": round(W * H * n2 / w2 / 1e9, 3),
                                  "kernel_ms": round(ms2, 4),
                                  "roofline_frac": round(bpc * W * H / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        w3, ms3, tiles, _ = run_rate(case, "masked", args.variant, args.rows, max(100, args.steps // 4),
-                                  args.warmup, args.timed_steps)
+        from fip_amd.sim import tune
+        mt = tune(case, mode="masked", variant=args.variant, steps=10)
         n3 = max(100, args.steps // 4)
+        w3, ms3, tiles, _ = run_rate(case, "masked", args.variant, mt["tile_rows"], n3, args.warmup,
+                                     args.timed_steps, launch=(mt["waves_per_block"], mt["blocks_per_cu"]))
         out["masked"] = {"gcells_effective": round(W * H * n3 / w3 / 1e9, 3),
                          "kernel_ms": round(ms3, 4), "tiles_active": tiles[0],
-                         "tiles_total": tiles[1]}
+                         "tiles_total": tiles[1],
+                         "tuned": {k: mt[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")}}
     if not args.no_extra and not args.no_ensemble:
         out["ensemble"] = run_ensemble(args.variant, args.ensemble_steps, 50)
     if not args.no_extra and not args.no_closure:
