@@ -113,15 +113,21 @@ def run_ensemble(variant, steps, warmup, nscn=64):
     return out
 
 
-def full_run(variant, chunk=5000, max_steps=0):
+# Launch shape of the whole 12 h run: whole runs measured 7.7 s with 16-row tiles and 2
+# wavefronts x 4 work-groups per CU, 8.0 s with 16 rows and 4 x 4, 8.6 s with 32 rows and
+# 2 x 4, and 12.3 s with the shape fip_tune picks on the 30 %-wet state alone (32 rows,
+# 4 x 16): most steps of the run have few active tiles, where short tiles finish sooner.
+FULL_RUN_SHAPE = {"tile_rows": 16, "waves_per_block": 2, "blocks_per_cu": 4}
+
+
+def full_run(variant, chunk=5000, max_steps=0, tuned=False):
     """The 5667 x 6500 workload over its whole 12 h from the dry start in masked mode:
     steps, wall time, effective rate (W*H per step) and the active-tile fraction sampled
-    every `chunk` steps. Tile rows and launch shape come from the autotuner run on the
-    30 %-wet state (the late part of the run, where the time goes). Returns a dict for a
-    JSON line."""
+    every `chunk` steps. The launch shape is FULL_RUN_SHAPE, or (tuned) the autotuner's
+    pick on the 30 %-wet state. Returns a dict for a JSON line."""
     from fip_amd.sim import Sim, tune
     from fip_amd.workloads import config1
-    best = tune(wet_state(config1()), mode="masked", variant=variant, steps=10)
+    best = tune(wet_state(config1()), mode="masked", variant=variant, steps=10) if tuned else dict(FULL_RUN_SHAPE)
     rows = best["tile_rows"]
     case = config1()
     valid = case.bed != np.float32(case.no_data)
@@ -147,7 +153,8 @@ def full_run(variant, chunk=5000, max_steps=0):
     vol = float(h.astype(np.float64)[valid].sum() * case.dx * case.dx)
     out = {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q hydrograph "
                        "0->600->0 m3/s over 12 h, right open, dry start, masked mode",
-           "variant": variant, "tuned": {k: best[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")},
+           "variant": variant, "shape": {k: best[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")},
+           "shape_from": "fip_tune on the 30 %-wet state" if tuned else "whole-run sweep",
            "simulated_s": round(s.time()[0], 2), "steps": steps,
            "wall_s": round(wall, 2), "ms_per_step": round(wall / steps * 1e3, 4),
            "gcells_effective": round(case.W * case.H * steps / wall / 1e9, 2),
@@ -263,6 +270,8 @@ def main():
     ap.add_argument("--ensemble-full-run", action="store_true",
                     help="run the 64-scenario ensemble from its dry start over its whole 6 h (masked) "
                          "and print that leg's JSON line only")
+    ap.add_argument("--full-run-tuned", action="store_true",
+                    help="full runs: take the launch shape from the autotuner on the 30%%-wet state")
     ap.add_argument("--full-run-steps", type=int, default=0,
                     help="stop the full run after this many steps (0: the whole 12 h)")
     ap.add_argument("--calibrate", action="store_true",
@@ -272,7 +281,8 @@ def main():
         calibrate_copy()
         return
     if args.full_run:
-        print(json.dumps(full_run(args.variant, max_steps=args.full_run_steps)), flush=True)
+        print(json.dumps(full_run(args.variant, max_steps=args.full_run_steps, tuned=args.full_run_tuned)),
+              flush=True)
         return
     if args.ensemble_full_run:
         print(json.dumps(ensemble_full_run(args.variant)), flush=True)
