@@ -319,7 +319,7 @@ def main():
         "config": {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q BC, "
                                "right open, 30% wet, dense mode",
                    "W": W, "H": H, "mode": "dense", "variant": args.variant,
-                   "tile": f"60 columns x {st['dense_span']}-row spans, 8 wavefronts in step",
+                   "tile": f"60 columns x {st['dense_span']}-row spans, 16 wavefronts in step",
                    "kst": f"{st['kst_classes']} classes, 1 B per cell" if st["kst_classes"] else "f32",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
