@@ -109,6 +109,7 @@ def run_ensemble(variant, steps, warmup, nscn=64):
                      "tuned": {k: best[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")},
                      "tune_seconds": round(tune_s, 1)}
         if mode == "dense":
+            out[mode]["span"] = st["dense_span"]
             out[mode]["roofline_frac"] = round(28 * cells / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     return out
 

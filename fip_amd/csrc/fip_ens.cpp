@@ -22,6 +22,12 @@ hipError_t fip_launch_ens_stencil_fast(const fip_ens_scn* scn, const unsigned lo
                                        fip_ens_ctl* ec, int par, unsigned cap, int tile_rows,
                                        int kvar, int blocks, int wpb, int persistent,
                                        hipStream_t st);
+hipError_t fip_launch_ens_dense_exact(const fip_ens_scn* scn, const int4* items, int nitems, int kvar, int wpb,
+                                      int par, hipStream_t st);
+hipError_t fip_launch_ens_dense_fast(const fip_ens_scn* scn, const int4* items, int nitems, int kvar, int wpb,
+                                     int par, hipStream_t st);
+hipError_t fip_ens_dense_waves_per_cu_exact(int kvar, int wpb, int* waves);
+hipError_t fip_ens_dense_waves_per_cu_fast(int kvar, int wpb, int* waves);
 hipError_t fip_launch_ens(int which, const fip_ens_scn* scn, int nscn, int total_tiles,
                           const fip_ens_bc* bcs, const float* series, const fip_ens_chunk* chunks,
                           int nchunks, fip_ens_ctl* ec, unsigned long long* list, int mode, int par,
@@ -60,13 +66,18 @@ struct fip_ens {
   fip_ens_scn* d_scn = nullptr;
   fip_ens_ctl* d_ctl = nullptr;
   unsigned long long* d_list = nullptr;  // [2][total_tiles]
+  int4* d_items = nullptr;  // dense mode: (scenario, first row, last row, first tile column)
+  int nitems = 0;
+  int items_wpb = 0;        // the wpb the items were planned for
+  int dense_span = 0;
+  std::vector<int> scn_tiles;  // tiles per scenario
   fip_ens_bc* d_bcs = nullptr;
   float* d_series = nullptr;
   std::vector<fip_ens_chunk*> d_chunks;  // per stage
   std::vector<int> nchunks;
   int total_tiles = 0;
   int cus = 256;
-  int wpb = 4;
+  int wpb = 4;  // set at creation: 8 for the dense fast variant, 4 otherwise
   int bpc = 0;  // persistent work-groups per CU (0 = one wavefront per list entry)
   int stencil_blocks = 0;
   int sched_blocks = 0;
@@ -113,6 +124,7 @@ int build(fip_ens* e) {
   std::vector<fip_ens_scn> scn(e->n);
   std::vector<std::vector<fip_ens_chunk>> chunks;
   int tile_base = 0;
+  e->scn_tiles.assign(e->n, 0);
   for (int s = 0; s < e->n; s++) {
     const fip_grid& g = fip_internal::grid(e->sims[s]);
     scn[s].g = g;
@@ -121,6 +133,7 @@ int build(fip_ens* e) {
     scn[s].tile_base = tile_base;
     scn[s].staggered = fip_internal::staggered(e->sims[s]);
     tile_base += g.tiles_x * g.tiles_y;
+    e->scn_tiles[s] = g.tiles_x * g.tiles_y;
     const ScnBCs& L = e->bcl[s];
     std::vector<int> stage, lim;
     fip_internal::bc_stages(L.bcs.data(), (int)L.bcs.size(), g.W, g.H, stage, lim);
@@ -176,7 +189,66 @@ int build(fip_ens* e) {
   return FIP_OK;
 }
 
+// Dense mode: the work items of the dense march. Every scenario is cut into spans of one
+// length, each span row into groups of wpb adjacent tile columns; the span is the
+// shortest (at least 16 rows, span + 2 a multiple of the 6-row register ring) for which
+// all items are resident at once. Items of a scenario are consecutive, span row by span
+// row, so that neighbouring groups (which share halo columns) land on one XCD.
+int plan_dense(fip_ens* e) {
+  int wpc = 0;
+  hipError_t r = e->fast ? fip_ens_dense_waves_per_cu_fast(e->kvar ? 1 : 0, e->wpb, &wpc)
+                         : fip_ens_dense_waves_per_cu_exact(e->kvar ? 1 : 0, e->wpb, &wpc);
+  if (r != hipSuccess) return fail(FIP_E_HIP, std::string("occupancy query: ") + hipGetErrorString(r));
+  if (wpc < 1) return fail(FIP_E_HIP, "the dense ensemble kernel cannot be resident");
+  const long slots = (long)e->cus * wpc;
+  std::vector<int> groups(e->n), heights(e->n);
+  for (int s = 0; s < e->n; s++) {
+    const fip_grid& g = fip_internal::grid(e->sims[s]);
+    groups[s] = (g.tiles_x + e->wpb - 1) / e->wpb;
+    heights[s] = g.H;
+  }
+  auto waves = [&](int span) {
+    long w = 0;
+    for (int s = 0; s < e->n; s++) w += (long)groups[s] * ((heights[s] + span - 1) / span) * e->wpb;
+    return w;
+  };
+  const int hmax = *std::max_element(heights.begin(), heights.end());
+  int span = 16;
+  while (waves(span) > slots && span < hmax) span += 6;
+  std::vector<int4> items;
+  for (int s = 0; s < e->n; s++)
+    for (int r0 = 0; r0 < heights[s]; r0 += span)
+      for (int gi = 0; gi < groups[s]; gi++)
+        items.push_back(make_int4(s, r0, std::min(r0 + span, heights[s]) - 1, gi * e->wpb));
+  if (e->d_items) hipFree(e->d_items);
+  e->d_items = nullptr;
+  EHIP(hipMalloc(&e->d_items, items.size() * sizeof(int4)));
+  EHIP(hipMemcpy(e->d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+  e->nitems = (int)items.size();
+  e->items_wpb = e->wpb;
+  e->dense_span = span;
+  e->stats.dense_span = span;
+  return FIP_OK;
+}
+
 int launch_stencil(fip_ens* e, int par) {
+  if (e->dense) {
+    if (e->items_wpb != e->wpb) return fail(FIP_E_STATE, "dense work items not planned");
+    if (e->timing) EHIP(hipEventRecord(e->ev0, e->stream));
+    hipError_t r = e->fast ? fip_launch_ens_dense_fast(e->d_scn, e->d_items, e->nitems, e->kvar, e->wpb, par, e->stream)
+                           : fip_launch_ens_dense_exact(e->d_scn, e->d_items, e->nitems, e->kvar, e->wpb, par, e->stream);
+    if (r != hipSuccess) return fail(FIP_E_HIP, std::string("ensemble dense stencil launch: ") + hipGetErrorString(r));
+    if (e->timing) {
+      EHIP(hipEventRecord(e->ev1, e->stream));
+      EHIP(hipEventSynchronize(e->ev1));
+      float ms = 0.f;
+      EHIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+      e->stats.stencil_ms += ms;
+      e->stats.timed_launches++;
+    }
+    e->stats.stencil_launches++;
+    return FIP_OK;
+  }
   unsigned long long* list = e->d_list + (size_t)par * std::max(1, e->total_tiles);
   if (e->timing) EHIP(hipEventRecord(e->ev0, e->stream));
   const unsigned cap = (unsigned)e->total_tiles;
@@ -208,7 +280,7 @@ int launch_post_apply(fip_ens* e, int mode, int par, int q) {
   for (int st = 0; st < nst; st++) {
     const int nc = st < (int)e->nchunks.size() ? e->nchunks[st] : 0;
     const fip_ens_chunk* ch = st < (int)e->d_chunks.size() ? e->d_chunks[st] : nullptr;
-    const int sched = st == 0 ? e->sched_blocks : 0;
+    const int sched = st == 0 && !e->dense ? e->sched_blocks : 0;  // dense mode keeps no list
     if (nc + sched == 0) continue;
     unsigned long long* list = e->d_list + (size_t)q * std::max(1, e->total_tiles);
     r = fip_launch_ens(1, e->d_scn, e->n, e->total_tiles, e->d_bcs, e->d_series, ch, nc, e->d_ctl,
@@ -284,6 +356,7 @@ int fip_ens_create(const fip_desc* descs, int n, fip_ens** out) {
   e->fast = descs[0].variant == FIP_FAST;
   e->dense = descs[0].mode == FIP_MODE_DENSE;
   e->kvar = kvar;
+  e->wpb = e->dense && e->fast ? 8 : 4;
   e->bcl.resize(n);
   auto cleanup = [&](int rc) {
     fip_ens_destroy(e);
@@ -329,6 +402,7 @@ int fip_ens_destroy(fip_ens* e) {
   destroy_graphs(e);
   free_tables(e);
   if (e->d_list) hipFree(e->d_list);
+  if (e->d_items) hipFree(e->d_items);
   if (e->d_scn) hipFree(e->d_scn);
   if (e->d_ctl) hipFree(e->d_ctl);
   if (e->ev0) hipEventDestroy(e->ev0);
@@ -384,7 +458,7 @@ int fip_ens_set_launch(fip_ens* e, int waves_per_block, int blocks_per_cu) {
   if (!e) return fail(FIP_E_INVALID, "null argument");
   if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu > 64)
     return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu at most 64");
-  e->wpb = waves_per_block == 0 ? 4 : waves_per_block;
+  e->wpb = waves_per_block != 0 ? waves_per_block : e->dense && e->fast ? 8 : 4;
   e->bpc = blocks_per_cu < 0 ? 0 : blocks_per_cu;
   destroy_graphs(e);
   return FIP_OK;
@@ -397,18 +471,26 @@ int fip_ens_tune(const fip_desc* descs, int n, const fip_bc* const* bcs, const i
   static const int kRows[] = {16, 32, 64};
   static const int kWaves[] = {1, 2, 4, 8};
   static const int kBlocksPerCu[] = {0, 4, 8, 16};
+  static const int kDenseWaves[] = {2, 4, 8, 16};
+  const bool dense = n > 0 && descs[0].mode == FIP_MODE_DENSE;
+  std::vector<int> rows(kRows, kRows + 3), waves(kWaves, kWaves + 4), bpcs(kBlocksPerCu, kBlocksPerCu + 4);
+  if (dense) {  // the dense march uses neither the tile rows nor persistent work-groups
+    rows.assign(1, descs[0].tile_rows == 0 ? 64 : descs[0].tile_rows);
+    waves.assign(kDenseWaves, kDenseWaves + 4);
+    bpcs.assign(1, 0);
+  }
   std::vector<fip_desc> dd(descs, descs + n);
   fip_tune_result best{};
   best.stencil_ms = 1e30;
   std::vector<double> far(n, 1e300);
-  for (int R : kRows) {
+  for (int R : rows) {
     for (auto& d : dd) d.tile_rows = R;
     fip_ens* e = nullptr;
     int rc = fip_ens_create(dd.data(), n, &e);
     for (int s = 0; s < n && !rc; s++) rc = fip_ens_set_bcs(e, s, bcs[s], nbcs[s]);
     if (!rc) rc = fip_ens_set_run(e, far.data());
-    for (int w : kWaves)
-      for (int bpc : kBlocksPerCu) {
+    for (int w : waves)
+      for (int bpc : bpcs) {
         if (rc) break;
         int act = 0;
         long long done = 0;
@@ -443,6 +525,7 @@ int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps
   EHIP(hipSetDevice(e->dev));
   int rc;
   if (!e->built && (rc = build(e))) return rc;
+  if (e->dense && e->items_wpb != e->wpb && (rc = plan_dense(e))) return rc;
   if (!e->started) {
     if ((rc = launch_post_apply(e, 0, (int)(e->iter & 1), (int)(e->iter & 1)))) return rc;
     e->started = true;
@@ -484,9 +567,15 @@ int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps
     e->stats.steps += k;
     if ((rc = count_active(&act))) return rc;
   }
-  unsigned cnt[2];
-  EHIP(hipMemcpy(cnt, (char*)e->d_ctl + offsetof(fip_ens_ctl, list_count), sizeof cnt, hipMemcpyDeviceToHost));
-  e->stats.tiles_computed = cnt[e->iter & 1];  // the list of the next step
+  if (e->dense) {  // every tile of every running scenario
+    long long t = 0;
+    for (int s = 0; s < e->n; s++) t += fin[s] ? 0 : e->scn_tiles[s];
+    e->stats.tiles_computed = t;
+  } else {
+    unsigned cnt[2];
+    EHIP(hipMemcpy(cnt, (char*)e->d_ctl + offsetof(fip_ens_ctl, list_count), sizeof cnt, hipMemcpyDeviceToHost));
+    e->stats.tiles_computed = cnt[e->iter & 1];  // the list of the next step
+  }
   for (fip_sim* s : e->sims)
     if ((rc = fip_internal::refresh(s))) return rc;
   if (active) *active = act;

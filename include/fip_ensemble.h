@@ -34,14 +34,16 @@ int fip_ens_set_run(fip_ens* e, const double* target_durations);
 int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps_done);
 int fip_ens_get_time(fip_ens* e, int s, double* duration, float* dt, int* iteration);
 int fip_ens_download(fip_ens* e, int s, int field, int side, float* dst);
-/* Launch shape of the ensemble stencil: wavefronts per work-group (1..16; 0 = 4) and
- * persistent work-groups per CU (1..64), or 0 for a grid of one wavefront per active-tile
- * entry (< 0 = default: 0). Results do not depend on it. */
+/* Launch shape of the ensemble stencil: wavefronts per work-group (1..16; 0 = the
+ * default: 8 for the dense fast variant, 4 otherwise) and persistent work-groups per CU
+ * (1..64), or 0 for a grid of one wavefront per active-tile entry (< 0 = default: 0).
+ * Dense mode marches spans of rows with all wavefronts of a work-group in step and uses
+ * no persistent work-groups. Results do not depend on it. */
 int fip_ens_set_launch(fip_ens* e, int waves_per_block, int blocks_per_cu);
 /* Autotuner of the ensemble (fip_tune for ensembles): times the ensemble stencil of
  * these scenarios (bcs[s] holds nbcs[s] BCs of scenario s) for every tile-row count and
  * launch shape over `steps` timed steps after as many warm-up steps, and returns the
- * fastest. */
+ * fastest. Dense mode tries 2, 4, 8 and 16 wavefronts per work-group only. */
 int fip_ens_tune(const fip_desc* descs, int n, const fip_bc* const* bcs, const int* nbcs,
                  int steps, fip_tune_result* out);
 /* steps, tiles_total (all scenarios), tiles_computed (last step), stencil timing. */

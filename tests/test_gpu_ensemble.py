@@ -121,6 +121,43 @@ def test_ensemble_launch_shapes_agree():
             assert t1 == t2 and same_bits(h1, h2) and same_bits(q1, q2), (wpb, bpc)
 
 
+@pytest.mark.parametrize("variant", ["exact", "fast"])
+def test_ensemble_dense_shapes_agree(variant):
+    """Dense march: any waves per work-group (so any span length) gives the same bits,
+    and each scenario matches its single-domain dense run (bits for the exact variant,
+    1e-5 for the fast one, whose single-domain kernel is compiled separately)."""
+    from fip_amd.sim import Ensemble
+    cases = scenarios()
+    ref = None
+    for wpb in (8, 1, 2, 16):
+        print(f"ensemble dense {variant} wpb={wpb}", flush=True)
+        e = Ensemble(cases, mode="dense", variant=variant, tile_rows=16)
+        e.set_launch(wpb, 0)
+        e.set_run([c.duration for c in cases])
+        active, _ = e.simulate(1_000_000)
+        assert active == 0
+        st = e.stats()
+        assert st["dense_span"] >= 16 and (st["dense_span"] + 2) % 6 == 0
+        got = [(e.time(k), [e.field(k, f) for f in ("h", "qx", "qy", "hmax")]) for k in range(len(cases))]
+        e.close()
+        if ref is None:
+            ref = got
+            continue
+        for k, ((t1, f1), (t2, f2)) in enumerate(zip(ref, got)):
+            assert t1 == t2, (wpb, k)
+            assert all(same_bits(a, b) for a, b in zip(f1, f2)), (wpb, k)
+    for k, c in enumerate(cases):
+        s = run_single(c, "dense", variant, 16)
+        assert s.time()[2] == ref[k][0][2], k
+        for f, got in zip(("h", "qx", "qy"), ref[k][1]):
+            if variant == "exact":
+                assert same_bits(s.field(f), got), (k, f)
+            else:
+                d = np.abs(s.field(f).astype(np.float64) - got).max()
+                assert d <= 1e-5, (k, f, d)
+        s.close()
+
+
 def test_ensemble_chunked_runs_match_one_run():
     """Stopping and resuming (non-graph chunks, then graph chunks) changes nothing."""
     from fip_amd.sim import Ensemble
