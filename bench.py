@@ -72,7 +72,7 @@ def run_rate(case, mode, variant, rows, steps, warmup, timed_steps, launch=None)
     return (t1 - t0), ms, tiles, s1
 
 
-def run_ensemble(variant, steps, warmup, nscn=64):
+def run_ensemble(variant, steps, warmup, nscn=64, modes=("dense", "masked")):
     """Batched ensemble of config-2 scenarios (sizes 256^2 .. 2048x1536), each started from a
     flat water level wetting 30 % of its cells; tile rows and launch shape from the ensemble
     autotuner, per mode. Returns a dict for the JSON line."""
@@ -81,7 +81,7 @@ def run_ensemble(variant, steps, warmup, nscn=64):
     cases = [wet_state(config2(s)) for s in range(nscn)]
     cells = sum(c.W * c.H for c in cases)
     out = {"scenarios": nscn, "cells": cells}
-    for mode in ("dense", "masked"):
+    for mode in modes:
         t0 = time.perf_counter()
         best = tune_ensemble(cases, mode=mode, variant=variant, steps=10)
         tune_s = time.perf_counter() - t0
@@ -275,6 +275,8 @@ def main():
                     help="full runs: take the launch shape from the autotuner on the 30%%-wet state")
     ap.add_argument("--full-run-steps", type=int, default=0,
                     help="stop the full run after this many steps (0: the whole 12 h)")
+    ap.add_argument("--ensemble-only", choices=["dense", "masked"],
+                    help="run only the batched-ensemble leg in this mode and print its result")
     ap.add_argument("--calibrate", action="store_true",
                     help="run 1 GiB device copies (counter calibration) and exit")
     args = ap.parse_args()
@@ -287,6 +289,10 @@ def main():
         return
     if args.ensemble_full_run:
         print(json.dumps(ensemble_full_run(args.variant)), flush=True)
+        return
+    if args.ensemble_only:
+        print(json.dumps(run_ensemble(args.variant, args.ensemble_steps, 50, modes=(args.ensemble_only,))),
+              flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the single-domain step does not shard "
