@@ -16,9 +16,15 @@
 //   aos       the same cells with h, qx, qy, bed interleaved as one float4 per cell: one
 //             16-byte read and one 16-byte write per cell (bed rewritten) plus the class
 //             byte, 33 B per cell, with plain or non-temporal stores;
-//   copy      a plain 16-byte-per-lane copy of one field (1 read + 1 write).
+//   copy      a plain 16-byte-per-lane copy of one field (1 read + 1 write);
+//   read      the flat stream's reads alone (4 floats and the class byte per cell);
+//   write     the flat stream's writes alone (3 floats per cell);
+//   copy4     four fields copied at once (4 reads + 4 writes: eight streams at 1:1);
+//   stagger   the flat stream with the eight fields in one allocation, field k starting
+//             k * (field size + s) bytes in, for a stagger s.
 // Every access stays inside its allocation: rows are below H, columns are clamped to
-// [0, W - 1], flat indices are below W * H / 4.
+// [0, W - 1], flat indices are below W * H / 4, and the staggered fields end inside the
+// allocation (checked on the host).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -121,6 +127,42 @@ __global__ __launch_bounds__(256) void aos(const float4* a, const unsigned char*
   }
 }
 
+__global__ __launch_bounds__(256) void rd(Fields f, size_t n4, float* sink) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const uchar4 cl = reinterpret_cast<const uchar4*>(f.cls)[i];
+    s.x += cl.x;
+    s.y += cl.w;
+#pragma unroll
+    for (int k = 0; k < kIn; k++) {
+      const float4 v = reinterpret_cast<const float4*>(f.in[k])[i];
+      s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    }
+  }
+  if (s.x + s.y + s.z + s.w == 1234.5f) sink[0] = s.x;  // keeps the loads; never true here
+}
+
+__global__ __launch_bounds__(256) void wr(Fields f, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const float4 v = make_float4((float)i, 1.f, 2.f, 3.f);
+#pragma unroll
+    for (int k = 0; k < kOut; k++) reinterpret_cast<float4*>(f.out[k])[i] = v;
+  }
+}
+
+// in[0..2] -> out[0..2] and in[3] -> o4
+__global__ __launch_bounds__(256) void copy4(Fields f, float* o4, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+#pragma unroll
+    for (int k = 0; k < kOut; k++)
+      reinterpret_cast<float4*>(f.out[k])[i] = reinterpret_cast<const float4*>(f.in[k])[i];
+    reinterpret_cast<float4*>(o4)[i] = reinterpret_cast<const float4*>(f.in[3])[i];
+  }
+}
+
 __global__ __launch_bounds__(256) void copy(const float4* a, float4* b, size_t n4) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) b[i] = a[i];
@@ -212,6 +254,42 @@ int main() {
   }, reps);
   std::printf("| copy | 1 read + 1 write, 8192 blocks | %.4f | %.0f (8 B per element) |\n", ms,
               8.0 * 4 * n4 / ms / 1e6);
+  float* sink = nullptr;
+  CK(hipMalloc(&sink, 256));
+  float* o4 = nullptr;
+  CK(hipMalloc(&o4, n * sizeof(float)));
+  float m = time_ms([&] { hipLaunchKernelGGL(rd, dim3(8192), dim3(256), 0, 0, f, n4, sink); }, reps);
+  std::printf("| read | 8192 blocks of 256 | %.4f | %.0f (17 B per cell moved: %.0f) |\n", m,
+              bytes / m / 1e6, 17.0 * 4 * n4 / m / 1e6);
+  m = time_ms([&] { hipLaunchKernelGGL(wr, dim3(8192), dim3(256), 0, 0, f, n4); }, reps);
+  std::printf("| write | 8192 blocks of 256 | %.4f | %.0f (12 B per cell moved: %.0f) |\n", m,
+              bytes / m / 1e6, 12.0 * 4 * n4 / m / 1e6);
+  m = time_ms([&] { hipLaunchKernelGGL(copy4, dim3(8192), dim3(256), 0, 0, f, o4, n4); }, reps);
+  std::printf("| copy4 | 4 reads + 4 writes, 8192 blocks | %.4f | %.0f (32 B per cell moved: %.0f) |\n", m,
+              bytes / m / 1e6, 32.0 * 4 * n4 / m / 1e6);
+  CK(hipFree(o4));
+  CK(hipFree(sink));
+  {
+    // eight fields of fb bytes (a 2 MiB multiple) in one allocation; field k at k (fb + s)
+    const size_t fb = (n * sizeof(float) + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+    const size_t smax = 1u << 20, total = 8 * (fb + smax);
+    char* big = nullptr;
+    CK(hipMalloc(&big, total));
+    CK(hipMemset(big, 0, total));
+    for (size_t sg : {(size_t)0, (size_t)4096, (size_t)65536, smax}) {
+      Fields h{};
+      size_t off = 0;
+      for (int k = 0; k < kIn; k++, off += fb + sg) h.in[k] = reinterpret_cast<const float*>(big + off);
+      h.cls = reinterpret_cast<const unsigned char*>(big + off);
+      off += fb + sg;
+      for (int k = 0; k < kOut; k++, off += fb + sg) h.out[k] = reinterpret_cast<float*>(big + off);
+      if (off > total) return 1;
+      m = time_ms([&] { hipLaunchKernelGGL(flat, dim3(8192), dim3(256), 0, 0, h, n4); }, reps);
+      std::printf("| stagger | field bases %zu B apart beyond their size | %.4f | %.0f |\n", sg, m,
+                  29.0 * 4 * n4 / m / 1e6);
+    }
+    CK(hipFree(big));
+  }
   CK(hipDeviceSynchronize());
   return 0;
 }

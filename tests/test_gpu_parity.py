@@ -54,6 +54,11 @@ CASES = {
     "kvar_many_110x75": dict(W=110, H=75, wet=True, kst_var="many"),
     "q_series_100x64": dict(W=100, H=64, wet=False, bc_kinds=("q", "open", "close", "close")),
     "all_open_91x77": dict(W=91, H=77, wet=True, bc_kinds=("open", "open", "open", "open")),
+    # right-hand edges: W mod 60 = 59, 0 and 58 (the last tile column full, empty, or one
+    # column short of full; lanes from W + 2 on load nothing)
+    "open_119x70": dict(W=119, H=70, wet=True, bc_kinds=("open", "open", "open", "open")),
+    "wet_120x66": dict(W=120, H=66, wet=True),
+    "kvar_178x61": dict(W=178, H=61, wet=True, kst_var=True),
 }
 
 
