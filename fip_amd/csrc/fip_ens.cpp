@@ -66,7 +66,7 @@ struct fip_ens {
   fip_ens_scn* d_scn = nullptr;
   fip_ens_ctl* d_ctl = nullptr;
   unsigned long long* d_list = nullptr;  // [2][total_tiles]
-  int4* d_items = nullptr;  // dense mode: (scenario, first row, last row, first tile column)
+  int4* d_items = nullptr;  // dense mode, per wavefront: (scenario, first row, last row, tile column)
   int nitems = 0;
   int items_wpb = 0;        // the wpb the items were planned for
   int dense_span = 0;
@@ -189,37 +189,66 @@ int build(fip_ens* e) {
   return FIP_OK;
 }
 
-// Dense mode: the work items of the dense march. Every scenario is cut into spans of one
-// length, each span row into groups of wpb adjacent tile columns; the span is the
-// shortest (at least 16 rows, span + 2 a multiple of the 6-row register ring) for which
-// all items are resident at once. Items of a scenario are consecutive, span row by span
-// row, so that neighbouring groups (which share halo columns) land on one XCD.
+// Dense mode: the work items of the dense march, one per wavefront. Every scenario is cut
+// into spans of one length; the tile columns of all span rows with the same first and
+// last row (the first span row of every scenario, the interior ones, the last ones of
+// scenarios of one height) are listed together, scenario by scenario and column by
+// column, and cut into work-groups of wpb wavefronts, the last one padded with
+// wavefronts past the grid. All wavefronts of a work-group then march the same rows, so
+// the barriers of the march stay matched; neighbouring work-groups (which share halo
+// columns) land on one XCD. The span is the shortest (at least 16 rows, span + 2 a
+// multiple of the 6-row register ring) for which every work-group is resident at once.
 int plan_dense(fip_ens* e) {
   int wpc = 0;
   hipError_t r = e->fast ? fip_ens_dense_waves_per_cu_fast(e->kvar ? 1 : 0, e->wpb, &wpc)
                          : fip_ens_dense_waves_per_cu_exact(e->kvar ? 1 : 0, e->wpb, &wpc);
   if (r != hipSuccess) return fail(FIP_E_HIP, std::string("occupancy query: ") + hipGetErrorString(r));
   if (wpc < 1) return fail(FIP_E_HIP, "the dense ensemble kernel cannot be resident");
-  const long slots = (long)e->cus * wpc;
-  std::vector<int> groups(e->n), heights(e->n);
+  const long slots = (long)e->cus * (wpc / e->wpb * e->wpb);
+  std::vector<int> tiles(e->n), heights(e->n);
   for (int s = 0; s < e->n; s++) {
     const fip_grid& g = fip_internal::grid(e->sims[s]);
-    groups[s] = (g.tiles_x + e->wpb - 1) / e->wpb;
+    tiles[s] = g.tiles_x;
     heights[s] = g.H;
   }
-  auto waves = [&](int span) {
+  // wave-columns per row range (first, last), in order of first appearance
+  auto ranges = [&](int span, std::vector<std::pair<std::pair<int, int>, std::vector<int2>>>& out) {
+    out.clear();
+    for (int s = 0; s < e->n; s++)
+      for (int r0 = 0; r0 < heights[s]; r0 += span) {
+        const std::pair<int, int> key(r0, std::min(r0 + span, heights[s]) - 1);
+        size_t k = 0;
+        while (k < out.size() && out[k].first != key) k++;
+        if (k == out.size()) out.push_back({key, {}});
+        for (int tx = 0; tx < tiles[s]; tx++) out[k].second.push_back(make_int2(s, tx));
+      }
+  };
+  std::vector<std::pair<std::pair<int, int>, std::vector<int2>>> rg;
+  auto waves = [&](int span) {  // the wavefronts of ranges(span), without listing them
+    std::vector<std::pair<std::pair<int, int>, long>> cnt;
+    for (int s = 0; s < e->n; s++)
+      for (int r0 = 0; r0 < heights[s]; r0 += span) {
+        const std::pair<int, int> key(r0, std::min(r0 + span, heights[s]) - 1);
+        size_t k = 0;
+        while (k < cnt.size() && cnt[k].first != key) k++;
+        if (k == cnt.size()) cnt.push_back({key, 0});
+        cnt[k].second += tiles[s];
+      }
     long w = 0;
-    for (int s = 0; s < e->n; s++) w += (long)groups[s] * ((heights[s] + span - 1) / span) * e->wpb;
+    for (auto& k : cnt) w += (k.second + e->wpb - 1) / e->wpb * e->wpb;
     return w;
   };
   const int hmax = *std::max_element(heights.begin(), heights.end());
   int span = 16;
   while (waves(span) > slots && span < hmax) span += 6;
+  ranges(span, rg);
   std::vector<int4> items;
-  for (int s = 0; s < e->n; s++)
-    for (int r0 = 0; r0 < heights[s]; r0 += span)
-      for (int gi = 0; gi < groups[s]; gi++)
-        items.push_back(make_int4(s, r0, std::min(r0 + span, heights[s]) - 1, gi * e->wpb));
+  for (auto& k : rg) {
+    const int first = k.first.first, last = k.first.second;
+    for (const int2& c : k.second) items.push_back(make_int4(c.x, first, last, c.y));
+    const int s0 = k.second.front().x;
+    while (items.size() % e->wpb) items.push_back(make_int4(s0, first, last, tiles[s0]));
+  }
   if (e->d_items) hipFree(e->d_items);
   e->d_items = nullptr;
   EHIP(hipMalloc(&e->d_items, items.size() * sizeof(int4)));
