@@ -143,11 +143,11 @@ extern "C" __global__ __launch_bounds__(256) void fip_ens_post(const fip_ens_scn
 }
 
 // Global active-tile schedule of list q: every tile of every running scenario whose
-// dilated mask is non-empty (all valid rows in dense mode), one atomic per wavefront.
+// dilated mask is non-empty (all valid rows in dense mode), one atomic per work-group and
+// pass (fip_block_append; every thread of the work-group runs the same passes).
 __device__ __forceinline__ void ens_schedule(const fip_ens_scn* scn, int nscn, int total,
                                             fip_ens_ctl* ec, unsigned long long* list, int q,
                                             int dense, int block, int nblocks) {
-  const int lane = threadIdx.x & 63;
   const int stride = nblocks * blockDim.x;
   for (int base = block * blockDim.x; base < total; base += stride) {
     const int gt = base + threadIdx.x;
@@ -174,18 +174,11 @@ __device__ __forceinline__ void ens_schedule(const fip_ens_scn* scn, int nscn, i
       }
     }
     const bool act = need != 0ull;
-    const unsigned long long bal = __ballot(act);
-    if (bal == 0ull) continue;
-    const int cnt = __popcll(bal);
-    const int before = __popcll(bal & ((1ull << lane) - 1ull));
-    unsigned pos0 = 0;
-    if (lane == 0) pos0 = atomicAdd(&ec->list_count[q], (unsigned)cnt);
-    pos0 = __shfl(pos0, 0);
-    if (act && pos0 + before < (unsigned)total) {
+    const unsigned pos = fip_block_append(act, &ec->list_count[q]);
+    if (act && pos < (unsigned)total) {
       const unsigned long long lo = (unsigned long long)__builtin_ctzll(need);
       const unsigned long long hi = 63ull - (unsigned long long)__builtin_clzll(need);
-      list[pos0 + before] =
-          ((unsigned long long)s << 40) | ((unsigned long long)t << 12) | (lo << 6) | hi;
+      list[pos] = ((unsigned long long)s << 40) | ((unsigned long long)t << 12) | (lo << 6) | hi;
     }
   }
 }

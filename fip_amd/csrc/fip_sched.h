@@ -29,27 +29,45 @@ __device__ __forceinline__ unsigned long long fip_tile_need(const fip_grid& g, i
   return need & fip_valid_rows(nrows);
 }
 
-// Appends the active tiles to list[q]; called by every thread of `nblocks` blocks.
-// Within a wavefront the order of tiles is kept; one atomic per wavefront.
+// Positions in a list of length *count for the threads of a work-group that pass act:
+// one atomic per work-group and call (one per wavefront put hundreds of atomics on the
+// one counter each step, and they serialise). Every thread of the work-group must make
+// the call, the same number of times (it contains barriers). Returns this thread's
+// position when act; positions follow the thread order within the work-group.
+__device__ __forceinline__ unsigned fip_block_append(bool act, unsigned* count) {
+  __shared__ unsigned s_cnt[16];
+  __shared__ unsigned s_base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  const unsigned long long bal = __ballot(act);
+  if (lane == 0) s_cnt[wave] = (unsigned)__popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tot = 0u;
+    for (int w = 0; w < nw; w++) tot += s_cnt[w];
+    s_base = tot ? atomicAdd(count, tot) : 0u;
+  }
+  __syncthreads();
+  unsigned pos = s_base;
+  for (int w = 0; w < wave; w++) pos += s_cnt[w];
+  pos += (unsigned)__popcll(bal & ((1ull << lane) - 1ull));
+  __syncthreads();  // s_cnt and s_base are rewritten by the next call
+  return pos;
+}
+
+// Appends the active tiles to list[q]; called by every thread of `nblocks` work-groups of
+// at most 1024 threads (every thread of a work-group runs the same iterations).
 __device__ __forceinline__ void fip_schedule(const fip_grid& g, int q, int block, int nblocks) {
   const int ntiles = g.tiles_x * g.tiles_y;
-  const int lane = threadIdx.x & 63;
   const int stride = nblocks * blockDim.x;
   for (int base = block * blockDim.x; base < ntiles; base += stride) {
     const int t = base + threadIdx.x;
-    unsigned long long need = t < ntiles ? fip_tile_need(g, t) : 0ull;
+    const unsigned long long need = t < ntiles ? fip_tile_need(g, t) : 0ull;
     const bool act = need != 0ull;
-    const unsigned long long bal = __ballot(act);
-    if (bal == 0ull) continue;
-    const int cnt = __popcll(bal);
-    const int before = __popcll(bal & ((1ull << lane) - 1ull));
-    unsigned pos0 = 0;
-    if (lane == 0) pos0 = atomicAdd(&g.ctl->list_count[q], (unsigned)cnt);
-    pos0 = __shfl(pos0, 0);
-    if (act && pos0 + before < (unsigned)ntiles) {
+    const unsigned pos = fip_block_append(act, &g.ctl->list_count[q]);
+    if (act && pos < (unsigned)ntiles) {
       const unsigned lo = (unsigned)__builtin_ctzll(need);
       const unsigned hi = 63u - (unsigned)__builtin_clzll(need);
-      g.list[q][pos0 + before] = ((unsigned)t << 12) | (lo << 6) | hi;
+      g.list[q][pos] = ((unsigned)t << 12) | (lo << 6) | hi;
     }
   }
 }
