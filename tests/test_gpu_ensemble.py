@@ -158,6 +158,30 @@ def test_ensemble_dense_shapes_agree(variant):
         s.close()
 
 
+@pytest.mark.parametrize("mode,rows", [("masked", 32), ("dense", 16)])
+def test_ensemble_kst_var_equals_single_domain(mode, rows):
+    """Scenarios with per-cell kSt (the ensemble reads it as floats; a single domain with
+    at most 64 distinct values reads a one-byte class): same bits, exact variant."""
+    from fip_amd.sim import Ensemble
+    cases = []
+    for k, (W, H) in enumerate(((97, 83), (130, 70), (64, 128))):
+        c = small_case(W, H, scenario=k, wet=True, kst_var=True, no_data=(k == 1))
+        c.duration = 20.0 + 6.0 * k
+        cases.append(c)
+    print(f"ensemble kst_var {mode}", flush=True)
+    e = Ensemble(cases, mode=mode, variant="exact", tile_rows=rows)
+    e.set_run([c.duration for c in cases])
+    active, _ = e.simulate(1_000_000)
+    assert active == 0
+    for k, c in enumerate(cases):
+        s = run_single(c, mode, "exact", rows)
+        assert s.time() == e.time(k), k
+        for f in ("h", "qx", "qy", "hmax"):
+            assert same_bits(s.field(f), e.field(k, f)), (k, f)
+        s.close()
+    e.close()
+
+
 def test_ensemble_chunked_runs_match_one_run():
     """Stopping and resuming (non-graph chunks, then graph chunks) changes nothing."""
     from fip_amd.sim import Ensemble

@@ -97,6 +97,36 @@ def test_dense_exact_device_resident(name):
     assert same_bits(g.field("hmax"), o.field("hmax"))
 
 
+@pytest.mark.parametrize("variant", ["exact", "fast"])
+def test_dense_flood_plains_across_stops(variant):
+    """The flood plains of the steps that reach iteration % 100 == 0 (src/main.cpp:125-128)
+    across device-resident runs that stop before, at and after multiples of 100 (and
+    graph-sized chunks between them): the same hmax as one uninterrupted run, and for the
+    exact variant the oracle's hmax and state at every stop."""
+    case = small_case(**CASES["wet_97x83"])
+    case.duration = 1e9
+    a = sim(case, variant=variant)
+    a.set_run(case.duration)
+    b = sim(case, variant=variant)
+    b.set_run(case.duration)
+    o = Oracle(case, DENSE) if variant == "exact" else None
+    total = 0
+    for n in (37, 63, 100, 100, 1, 99, 200, 150, 50):
+        print(f"flood {variant}: {n} steps from {total}", flush=True)
+        ev, done = b.simulate(n)
+        assert done == n
+        total += n
+        if o is not None:
+            for _ in range(n):
+                o.step()
+            assert same_bits(b.field("hmax"), o.field("hmax")), total
+            assert_state_equal(b, o, f"flood stop {total}")
+    a.simulate(total)
+    assert a.time() == b.time()
+    assert same_bits(a.field("hmax"), b.field("hmax"))
+    assert same_bits(a.field("h"), b.field("h"))
+
+
 @pytest.mark.parametrize("rows", [16, 32, 64])
 @pytest.mark.parametrize("name", ["wet_97x83", "dry_z_inflow_130x70", "kvar_nodata_121x90",
                                   "kvar_many_110x75"])
