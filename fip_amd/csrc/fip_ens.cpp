@@ -77,7 +77,7 @@ struct fip_ens {
   std::vector<int> nchunks;
   int total_tiles = 0;
   int cus = 256;
-  int wpb = 4;  // set at creation: 8 for the dense fast variant, 4 otherwise
+  int wpb = 4;  // set at creation: 16 for the dense fast variant, 4 otherwise
   int bpc = 0;  // persistent work-groups per CU (0 = one wavefront per list entry)
   int stencil_blocks = 0;
   int sched_blocks = 0;
@@ -385,7 +385,7 @@ int fip_ens_create(const fip_desc* descs, int n, fip_ens** out) {
   e->fast = descs[0].variant == FIP_FAST;
   e->dense = descs[0].mode == FIP_MODE_DENSE;
   e->kvar = kvar;
-  e->wpb = e->dense && e->fast ? 8 : 4;
+  e->wpb = e->dense && e->fast ? 16 : 4;
   e->bcl.resize(n);
   auto cleanup = [&](int rc) {
     fip_ens_destroy(e);
@@ -487,7 +487,7 @@ int fip_ens_set_launch(fip_ens* e, int waves_per_block, int blocks_per_cu) {
   if (!e) return fail(FIP_E_INVALID, "null argument");
   if (waves_per_block < 0 || waves_per_block > 16 || blocks_per_cu > 64)
     return fail(FIP_E_INVALID, "waves_per_block must be 0..16 and blocks_per_cu at most 64");
-  e->wpb = waves_per_block != 0 ? waves_per_block : e->dense && e->fast ? 8 : 4;
+  e->wpb = waves_per_block != 0 ? waves_per_block : e->dense && e->fast ? 16 : 4;
   e->bpc = blocks_per_cu < 0 ? 0 : blocks_per_cu;
   destroy_graphs(e);
   return FIP_OK;

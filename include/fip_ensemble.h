@@ -35,7 +35,7 @@ int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps
 int fip_ens_get_time(fip_ens* e, int s, double* duration, float* dt, int* iteration);
 int fip_ens_download(fip_ens* e, int s, int field, int side, float* dst);
 /* Launch shape of the ensemble stencil: wavefronts per work-group (1..16; 0 = the
- * default: 8 for the dense fast variant, 4 otherwise) and persistent work-groups per CU
+ * default: 16 for the dense fast variant, 4 otherwise) and persistent work-groups per CU
  * (1..64), or 0 for a grid of one wavefront per active-tile entry (< 0 = default: 0).
  * Dense mode marches spans of rows with all wavefronts of a work-group in step and uses
  * no persistent work-groups. Results do not depend on it. */
