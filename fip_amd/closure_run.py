@@ -128,7 +128,7 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
     """Plain fine vs plain coarse vs learned coarse on held-out scenarios, as ensembles.
     Returns metrics and timings."""
     from . import metrics as co
-    from .sim import Ensemble
+    from .sim import Ensemble, tune_ensemble
     fines = [fine_case(s, W=W, H=W, duration=duration) for s in scenarios]
     pairs = [coarse_of(f) for f in fines]
     coarses = [p[0] for p in pairs]
@@ -138,7 +138,11 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
            "coarse": f"{coarses[0].W}x{coarses[0].H} dx=4"}
 
     def run(cases, closure):
+        # each ensemble runs with the launch shape the ensemble autotuner picks for it
+        best = tune_ensemble(cases, mode="dense", variant=variant, steps=10)
         e = Ensemble(cases, mode="dense", variant=variant, tile_rows=32)
+        e.set_launch(best["waves_per_block"], 0)
+        out.setdefault("waves_per_block_fine_plain_learned", []).append(best["waves_per_block"])
         if closure is not None:
             closure.attach_ensemble(e, bstd, zref, sat=sat)
         e.set_run([duration] * len(cases))
