@@ -207,6 +207,48 @@ def ensemble_full_run(variant, nscn=64, chunk=2000):
     return out
 
 
+def config0_run():
+    """The 512^2 workload that both the GPU and the oracle finish (dry start, left z
+    inflow, 3600 s): whole runs on the GPU (masked and dense, fast and exact) and on the
+    host (the binary32 oracle at the GPU's tile geometry, masked, and dense), wall time and
+    effective Gcell/s = W H steps / wall. The exact GPU runs must take the oracle's steps."""
+    from fip_amd.sim import Sim
+    from fip_amd.workloads import config0
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import DENSE, TILED, Oracle
+    case = config0()
+    W, H = case.W, case.H
+    out = {"workload": "plain single domain 512x512 dx=3, dry start, left z inflow, 3600 s",
+           "gpu": {}, "cpu": {}}
+    from fip_amd.sim import tune
+    shape = tune(case, mode="masked", variant="fast", steps=10)  # masked tile rows and launch shape
+    out["masked_shape"] = {k: shape[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")}
+    for mode, variant in (("masked", "fast"), ("masked", "exact"), ("dense", "fast"), ("dense", "exact")):
+        s = Sim(case, mode=mode, variant=variant, tile_rows=shape["tile_rows"])
+        if mode == "masked":
+            s.set_launch(shape["waves_per_block"], shape["blocks_per_cu"])
+        s.set_run(case.duration)
+        t0 = time.perf_counter()
+        ev, done = s.simulate(10 ** 7)
+        wall = time.perf_counter() - t0
+        s.close()
+        out["gpu"][f"{mode}_{variant}"] = {"steps": done, "wall_s": round(wall, 3),
+                                            "gcells_effective": round(W * H * done / wall / 1e9, 3)}
+    threads = min(16, os.cpu_count() or 1)
+    for name, o in (("masked", Oracle(case, TILED, tile_w=60, tile_rows=shape["tile_rows"], threads=threads)),
+                    ("dense", Oracle(case, DENSE, threads=threads))):
+        t0 = time.perf_counter()
+        while o.time()[0] < case.duration:
+            o.step()
+        wall = time.perf_counter() - t0
+        steps = o.time()[2]
+        out["cpu"][name] = {"steps": steps, "wall_s": round(wall, 3), "threads": threads,
+                            "gcells_effective": round(W * H * steps / wall / 1e9, 4)}
+    out["exact_steps_match_oracle"] = (out["gpu"]["masked_exact"]["steps"] == out["cpu"]["masked"]["steps"]
+                                       and out["gpu"]["dense_exact"]["steps"] == out["cpu"]["dense"]["steps"])
+    return out
+
+
 def measured_traffic(variant, rows):
     """HBM bytes per stencil launch from the committed rocprofv3 --pmc summary of this
     workload (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
@@ -277,6 +319,8 @@ def main():
                     help="stop the full run after this many steps (0: the whole 12 h)")
     ap.add_argument("--ensemble-only", choices=["dense", "masked"],
                     help="run only the batched-ensemble leg in this mode and print its result")
+    ap.add_argument("--config0", action="store_true",
+                    help="whole runs of the 512x512 workload on the GPU and the oracle; print that line only")
     ap.add_argument("--calibrate", action="store_true",
                     help="run 1 GiB device copies (counter calibration) and exit")
     args = ap.parse_args()
@@ -289,6 +333,9 @@ def main():
         return
     if args.ensemble_full_run:
         print(json.dumps(ensemble_full_run(args.variant)), flush=True)
+        return
+    if args.config0:
+        print(json.dumps(config0_run()), flush=True)
         return
     if args.ensemble_only:
         print(json.dumps(run_ensemble(args.variant, args.ensemble_steps, 50, modes=(args.ensemble_only,))),
