@@ -8,7 +8,10 @@
 //
 // Options the reference does not have are environment variables:
 //   FIP_MODE=masked|dense (default masked), FIP_VARIANT=exact|fast (default exact),
-//   FIP_TILE_ROWS=16|32|64 (default 32), FIP_DEVICE=<ordinal> (default 0).
+//   FIP_TILE_ROWS=16|32|64 (default 16), FIP_DEVICE=<ordinal> (default 0),
+//   FIP_WAVES_PER_BLOCK, FIP_BLOCKS_PER_CU (the stencil's launch shape; in masked mode
+//   default 2 and 4: persistent work-groups, the fastest shape of a whole 12 h run of the
+//   5667 x 6500 workload). Results depend on the mode, variant and tile rows only.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -115,10 +118,14 @@ int main(int argc, char** argv) {
   d.staggered = c.solver == 0 ? 1 : 0;
   d.mode = env_str("FIP_MODE", "masked") == "dense" ? FIP_MODE_DENSE : FIP_MODE_MASKED;
   d.variant = env_str("FIP_VARIANT", "exact") == "fast" ? FIP_FAST : FIP_EXACT;
-  d.tile_rows = env_int("FIP_TILE_ROWS", 32);
+  d.tile_rows = env_int("FIP_TILE_ROWS", 16);
   d.device = env_int("FIP_DEVICE", 0);
   fip_sim* sim = nullptr;
   check(fip_create(&d, &sim), "fip_create");
+  if (d.mode == FIP_MODE_MASKED)
+    check(fip_set_launch(sim, env_int("FIP_WAVES_PER_BLOCK", 2), env_int("FIP_BLOCKS_PER_CU", 4)), "fip_set_launch");
+  else
+    check(fip_set_launch(sim, env_int("FIP_WAVES_PER_BLOCK", 0), -1), "fip_set_launch");
 
   // BoundaryConditionKernel::generate (src/main.cpp:224)
   std::vector<fip_bc> bcs;
