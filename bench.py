@@ -249,6 +249,44 @@ def config0_run():
     return out
 
 
+def cli_run(variant):
+    """The drop-in end to end: writes the 5667x6500 12 h workload (rasters, CSV series,
+    parameters.yml, sampling of three points every 10 min) to a scratch folder and runs
+    `fip_amd/bin/fip parameters.yml` on it with the CLI's default options, timing the
+    process (file reading, set-up, the 217 638 steps, the sampling output)."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    from fip_amd.workloads import config1, write_case
+    case = config1()
+    d = tempfile.mkdtemp(prefix="fip_cli_")
+    try:
+        pts = [{"x": 200, "y": case.H // 2, "label": "h_in", "type": "h"},
+               {"x": case.W // 2, "y": case.H // 2, "label": "z_mid", "type": "z"},
+               {"x": case.W - 200, "y": case.H // 2, "label": "qx_out", "type": "qx"}]
+        p = write_case(case, d, sampling=pts, sampling_interval=600)
+        env = dict(os.environ)
+        if variant:
+            env["FIP_VARIANT"] = variant
+        t0 = time.perf_counter()
+        r = subprocess.run([os.path.join(ROOT, "fip_amd", "bin", "fip"), p], capture_output=True,
+                           text=True, timeout=900, env=env, cwd=d)
+        wall = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"fip exited with {r.returncode}: {r.stderr[-2000:]}")
+    m = re.search(r"Finished (\d+) iterations after ([0-9.eE+-]+) s", r.stderr)
+    rows = r.stdout.strip().splitlines()
+    return {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q hydrograph, right open, "
+                        "dry start, 12 h, through the fip CLI",
+            "variant": variant or "exact", "process_wall_s": round(wall, 2),
+            "iterations": int(m.group(1)) if m else None,
+            "reported_s": float(m.group(2)) if m else None,
+            "sample_rows": len(rows) - 1, "last_sample": rows[-1] if rows else None}
+
+
 def measured_traffic(variant, rows):
     """HBM bytes per stencil launch from the committed rocprofv3 --pmc summary of this
     workload (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
@@ -319,6 +357,8 @@ def main():
                     help="stop the full run after this many steps (0: the whole 12 h)")
     ap.add_argument("--ensemble-only", choices=["dense", "masked"],
                     help="run only the batched-ensemble leg in this mode and print its result")
+    ap.add_argument("--cli-run", action="store_true",
+                    help="run the fip CLI end to end on the 5667x6500 12 h workload; print that line only")
     ap.add_argument("--config0", action="store_true",
                     help="whole runs of the 512x512 workload on the GPU and the oracle; print that line only")
     ap.add_argument("--calibrate", action="store_true",
@@ -333,6 +373,10 @@ def main():
         return
     if args.ensemble_full_run:
         print(json.dumps(ensemble_full_run(args.variant)), flush=True)
+        return
+    if args.cli_run:
+        # the CLI's own default variant (exact) unless --variant is given
+        print(json.dumps(cli_run(args.variant if "--variant" in sys.argv else None)), flush=True)
         return
     if args.config0:
         print(json.dumps(config0_run()), flush=True)
