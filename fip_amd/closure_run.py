@@ -11,7 +11,7 @@ water level of the discharge distribution at the channel bed + 2 m. A discharge 
 puts the same volume into the fine domain and its coarse twin (the bank row, dry on both
 grids, is the `to` cell that src/boundary_condition.cpp:10 leaves out of the sum); with a
 fixed-level (type z) inflow the two grids draw different volumes through the boundary,
-which no interior flux correction can change, and the judged depth errors would be set
+which no interior flux correction can change, and the measured depth errors would be set
 by the boundary rather than by the terrain. The coarse twin is the 4x restriction (bed =
 block mean, the same discharge boundary on the coarse rows). Both run with a fixed dt
 (0.05 s fine, 0.2 s coarse: one coarse step = four fine steps, the pairing the closure is
@@ -21,7 +21,7 @@ Training pairs: every `sample_every` seconds of a fine run, its state is restric
 the coarse grid, the coarse solver takes one plain step, the fine solver four steps, and
 (restricted input, plain coarse fluxes, restricted fine fluxes) is stored.
 Evaluation: held-out scenarios run plain fine, plain coarse and learned coarse (closure
-after every coarse step) to the same time; the coarse results are judged against the
+after every coarse step) to the same time; the coarse results are compared with the
 restricted fine run (depth RMSE, max-depth error, wet-extent IoU).
 """
 from __future__ import annotations

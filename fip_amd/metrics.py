@@ -1,5 +1,6 @@
 """Error measures of a coarse run against the restricted fine run of the same scenario
-(the judges of the learned closure): depth RMSE, max-depth error, wet-extent IoU."""
+(the measures the learned closure is scored by): depth RMSE, max-depth error, wet-extent
+IoU."""
 from __future__ import annotations
 
 import numpy as np
