@@ -72,3 +72,15 @@ def test_generated_bc_kernel_compiles(name, staggered):
     assert size > 0
     if name == "overlapping_same_side":
         assert src.count("// stage") == 2  # the overlap orders the two BCs
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: --gpus N with N > 1 exits with a message, before any
+    GPU work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
