@@ -95,7 +95,13 @@ struct fip_sim {
   bool started = false;
   bool run_set = false;
   int npoints = 0;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // captured chunks: graph of `steps` steps starting at parity `par` (plus the flood-plain
+  // kernel, which tests iteration % 100 itself), most recently built last
+  struct Chunk {
+    int par, steps;
+    hipGraphExec_t exec;
+  };
+  std::vector<Chunk> graphs;
   bool graph_failed = false;
   fip_internal::StepHook post_hook;  // extra work after each step of the device-resident loop
   // stats / timing
@@ -499,11 +505,8 @@ int compile_bc(fip_sim* s, const std::string& src) {
 }
 
 void destroy_graphs(fip_sim* s) {
-  for (auto& gx : s->graph)
-    if (gx) {
-      hipGraphExecDestroy(gx);
-      gx = nullptr;
-    }
+  for (auto& c : s->graphs) hipGraphExecDestroy(c.exec);
+  s->graphs.clear();
 }
 
 int one_step(fip_sim* s, int par) {
@@ -514,20 +517,61 @@ int one_step(fip_sim* s, int par) {
   return s->post_hook ? s->post_hook(s->stream, par) : FIP_OK;
 }
 
-// Captures kStepsPerGraph steps starting at parity par, plus the flood-plain kernel.
-int build_graph(fip_sim* s, int par) {
+// Captured chunks kept per simulation: a run uses at most three lengths (the steps up to
+// the next multiple of kStepsPerGraph, whole chunks, the rest), each at one or two parities.
+constexpr size_t kMaxGraphs = 8;
+
+// The graph of `steps` steps (1..kStepsPerGraph) starting at parity par, plus the
+// flood-plain kernel; captured and instantiated on first use. nullptr when capture failed.
+hipGraphExec_t chunk_graph(fip_sim* s, int par, int steps) {
+  for (auto& c : s->graphs)
+    if (c.par == par && c.steps == steps) return c.exec;
   hipGraph_t graph;
-  HIPTRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
   int rc = FIP_OK;
-  for (int k = 0; k < kStepsPerGraph && rc == FIP_OK; k++) rc = one_step(s, par ^ (k & 1));
+  for (int k = 0; k < steps && rc == FIP_OK; k++) rc = one_step(s, par ^ (k & 1));
   if (rc == FIP_OK) rc = launch_misc(s, 3, 1, 0);
-  hipError_t e = hipStreamEndCapture(s->stream, &graph);
-  if (rc != FIP_OK) return rc;
-  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&s->graph[par], graph, nullptr, nullptr, 0);
+  const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (rc != FIP_OK || e != hipSuccess) return nullptr;
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
-  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-  return FIP_OK;
+  if (ei != hipSuccess) return nullptr;
+  if (s->graphs.size() == kMaxGraphs) {
+    hipGraphExecDestroy(s->graphs.front().exec);
+    s->graphs.erase(s->graphs.begin());
+  }
+  s->graphs.push_back({par, steps, exec});
+  return exec;
+}
+
+// The chunks fip_simulate cuts `nsteps` steps from iteration it0 into: up to the next
+// multiple of kStepsPerGraph (the flood-plain cadence), then whole chunks. Calls
+// f(parity, steps) for each distinct chunk in order.
+template <class F>
+void for_each_chunk(int it0, long long nsteps, F f) {
+  long long it = it0, left = nsteps;
+  int seen = 0;
+  int last_par = -1, last_k = -1;
+  while (left > 0 && seen < 6) {
+    const long long k = std::min<long long>(kStepsPerGraph - it % kStepsPerGraph, left);
+    const int par = (int)(it & 1);
+    if (par != last_par || k != last_k) {
+      f(par, (int)k);
+      seen++;
+      last_par = par;
+      last_k = (int)k;
+    }
+    // whole chunks keep their parity (kStepsPerGraph is even): skip to the last one
+    if (k == kStepsPerGraph && left >= 2 * kStepsPerGraph) {
+      const long long whole = left / kStepsPerGraph;
+      it += (whole - 1) * kStepsPerGraph;
+      left -= (whole - 1) * kStepsPerGraph;
+      continue;
+    }
+    it += k;
+    left -= k;
+  }
 }
 
 int validate_bc(const fip_desc& d, const fip_bc& b) {
@@ -946,15 +990,16 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
     const int it0 = s->host_iter;
     const long long to_flood = kStepsPerGraph - (it0 % kStepsPerGraph);
     const long long k = std::min<long long>(to_flood, nsteps - done);
-    const bool use_graph = k == kStepsPerGraph && !s->timing && !s->graph_failed;
-    if (use_graph && !s->graph[it0 & 1]) {
-      if (build_graph(s, it0 & 1) != FIP_OK) {
+    hipGraphExec_t gx = nullptr;
+    if (!s->timing && !s->graph_failed) {
+      gx = chunk_graph(s, it0 & 1, (int)k);
+      if (!gx) {
         s->graph_failed = true;
         hipGetLastError();
       }
     }
-    if (use_graph && s->graph[it0 & 1]) {
-      HIPTRY(hipGraphLaunch(s->graph[it0 & 1], s->stream));
+    if (gx) {
+      HIPTRY(hipGraphLaunch(gx, s->stream));
       s->stats.stencil_launches += k;
     } else {
       for (long long i = 0; i < k; i++)
@@ -992,6 +1037,24 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
     s->stats.tiles_computed = cnt[s->host_iter & 1];
   } else {
     s->stats.tiles_computed = s->ntiles;
+  }
+  return FIP_OK;
+}
+
+int fip_prepare(fip_sim* s, long long nsteps) {
+  if (!s) return fail(FIP_E_INVALID, "null argument");
+  if (!s->run_set) return fail(FIP_E_STATE, "call fip_set_run first");
+  if (!s->bc_ready) return fail(FIP_E_STATE, "boundary kernel not generated (call fip_bc_generate)");
+  if (nsteps < 0) return fail(FIP_E_INVALID, "nsteps must not be negative");
+  HIPTRY(hipSetDevice(s->dev));
+  if (s->timing || s->graph_failed) return FIP_OK;
+  bool ok = true;
+  for_each_chunk(s->host_iter, nsteps, [&](int par, int k) {
+    if (ok && !chunk_graph(s, par, k)) ok = false;
+  });
+  if (!ok) {
+    s->graph_failed = true;
+    hipGetLastError();
   }
   return FIP_OK;
 }

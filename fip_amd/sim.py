@@ -18,7 +18,7 @@ EV_NONE, EV_FINISHED, EV_DUMP = 0, 1, 2
 EXPORTS = ["fip_last_error", "fip_create", "fip_destroy", "fip_bc_generate", "fip_bc_source",
            "fip_bc_launch",
            "fip_bc_apply", "fip_bc_params", "fip_control_timestep", "fip_compute_step",
-           "fip_floodplain", "fip_set_run", "fip_simulate", "fip_read_samples", "fip_get_time",
+           "fip_floodplain", "fip_set_run", "fip_simulate", "fip_prepare", "fip_read_samples", "fip_get_time",
            "fip_set_dt", "fip_download", "fip_upload", "fip_tiles", "fip_get_masks",
            "fip_get_cfl", "fip_set_launch", "fip_tune", "fip_set_timing", "fip_get_stats",
            "fip_sync"]
@@ -91,6 +91,7 @@ def lib():
         L.fip_set_run.argtypes = [vp, C.c_double, C.c_int, ip, ip, ip, C.c_int,
                                   C.POINTER(C.c_double), C.c_int]
         L.fip_simulate.argtypes = [vp, C.c_longlong, ip, C.POINTER(C.c_longlong)]
+        L.fip_prepare.argtypes = [vp, C.c_longlong]
         L.fip_read_samples.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_float), C.c_int, ip]
         L.fip_get_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_float), ip]
         L.fip_set_dt.argtypes = [vp, C.c_float]
@@ -358,6 +359,10 @@ class Sim:
         ev, done = C.c_int(), C.c_longlong()
         _check(lib().fip_simulate(self.s, int(nsteps), C.byref(ev), C.byref(done)))
         return ev.value, done.value
+
+    def prepare(self, nsteps):
+        """Capture the graphs that simulate(nsteps) launches next (nothing runs)."""
+        _check(lib().fip_prepare(self.s, int(nsteps)))
 
     def read_samples(self, max_rows=256):
         t = np.zeros(max_rows, dtype=np.float64)

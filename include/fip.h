@@ -152,6 +152,13 @@ int fip_set_run(fip_sim* sim, double target_duration, int sampling_interval,
                 const int* sample_x, const int* sample_y, const int* sample_type, int npoints,
                 const double* save_times, int nsave);
 int fip_simulate(fip_sim* sim, long long nsteps, int* event, long long* steps_done);
+/* Captures, without running anything, the hipGraphs that the next fip_simulate(sim,
+ * nsteps) call launches (its steps are cut at multiples of 100 iterations, the flood-plain
+ * cadence of src/main.cpp:125-128, and each chunk of a length and parity is captured once
+ * and kept). fip_simulate captures a chunk on first use anyway; calling this beforehand
+ * takes the capture out of the first call. No counterpart in the reference, whose loop
+ * launches every kernel from the host. */
+int fip_prepare(fip_sim* sim, long long nsteps);
 /* Drain logged samples: t[row], vals[row * npoints + p]; returns rows in *nrows. */
 int fip_read_samples(fip_sim* sim, double* t, float* vals, int max_rows, int* nrows);
 
