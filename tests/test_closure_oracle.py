@@ -76,6 +76,61 @@ def test_backward_matches_finite_differences():
     assert np.all(g[~co.live_mask()] == 0)
 
 
+def test_conv_tap_order_against_correlate2d():
+    # tap t = (dy + 1) * 3 + (dx + 1) reads the input at (y + dy, x + dx): for one input and
+    # one output channel that is scipy's 2-D cross-correlation with zero fill, kernel
+    # k[dy + 1, dx + 1] = w[t]; summed over channels for the multi-channel case
+    from scipy.signal import correlate2d
+    rng = np.random.default_rng(8)
+    N, H, W, Ci, Co = 2, 7, 9, 3, 4
+    x = rng.normal(0, 1, (N, H, W, Ci))
+    w = rng.normal(0, 1, (co.TAPS, Ci, Co))
+    b = rng.normal(0, 1, Co)
+    got = co.conv(x, w, b)
+    want = np.zeros((N, H, W, Co))
+    for n in range(N):
+        for o in range(Co):
+            want[n, :, :, o] = b[o]
+            for c in range(Ci):
+                k = w[:, c, o].reshape(3, 3)
+                want[n, :, :, o] += correlate2d(x[n, :, :, c], k, mode="same", boundary="fill")
+    assert np.abs(got - want).max() < 1e-12
+
+
+def test_conv_tap_order_brute_force_delta():
+    # a unit input at (y0, x0) and a single nonzero tap (dy, dx) give a single output at
+    # (y0 - dy, x0 - dx); at the image edge a tap that would read outside gives nothing
+    H, W = 6, 8
+    for t in range(co.TAPS):
+        dy, dx = t // 3 - 1, t % 3 - 1
+        w = np.zeros((co.TAPS, 1, 1))
+        w[t, 0, 0] = 2.0
+        for (y0, x0) in ((3, 4), (0, 0), (H - 1, W - 1)):
+            x = np.zeros((1, H, W, 1))
+            x[0, y0, x0, 0] = 1.0
+            out = co.conv(x, w)[0, :, :, 0]
+            nz = np.argwhere(out != 0)
+            y, xx = y0 - dy, x0 - dx
+            if 0 <= y < H and 0 <= xx < W:
+                assert nz.tolist() == [[y, xx]] and out[y, xx] == 2.0, (t, y0, x0)
+            else:
+                assert len(nz) == 0, (t, y0, x0)
+
+
+def test_conv_input_and_weight_grads_are_adjoints():
+    # <conv(x, w), dz> = <x, conv_input_grad(dz, w)> = <w, conv_weight_grad(x, dz)>: the
+    # two backward convolutions are the adjoints of the forward one, tap order included
+    rng = np.random.default_rng(9)
+    x = rng.normal(0, 1, (2, 5, 7, 3))
+    w = rng.normal(0, 1, (co.TAPS, 3, 4))
+    dz = rng.normal(0, 1, (2, 5, 7, 4))
+    lhs = (co.conv(x, w) * dz).sum()
+    assert abs(lhs - (x * co.conv_input_grad(dz, w)).sum()) < 1e-10
+    dw, db = co.conv_weight_grad(x, dz)
+    assert abs(lhs - (w * dw).sum()) < 1e-10
+    assert np.allclose(db, dz.reshape(-1, 4).sum(axis=0))
+
+
 def test_emulated_gradient_tracks_exact():
     rng = np.random.default_rng(6)
     N, H, W = 2, 12, 10
