@@ -49,11 +49,8 @@ def stored_bytes_per_cell(kvar: bool, kst_classes: int) -> int:
 
 def wet_state(case, frac=0.30):
     """Flat water level that wets `frac` of the valid cells (the late state of the run)."""
-    valid = case.bed != np.float32(case.no_data if case.no_data is not None else -9999.0)
-    level = np.float32(np.quantile(case.bed[valid], frac))
-    h = np.where(valid, np.maximum(np.float32(0), level - case.bed), np.float32(0)).astype(np.float32)
-    case.h = h
-    return case
+    from fip_amd.workloads import level_state
+    return level_state(case, frac)
 
 
 def run_rate(case, mode, variant, rows, steps, warmup, timed_steps, launch=None):
@@ -338,6 +335,20 @@ def calibrate_copy(nbytes=1 << 30, reps=3):
     print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def masked_leg(case, args):
+    """Masked mode on the headline workload with the autotuner's shape: effective rate
+    (W*H per step over wall time), stencil time, active tiles."""
+    from fip_amd.sim import tune
+    mt = tune(case, mode="masked", variant=args.variant, steps=10)
+    n3 = max(100, args.steps // 4)
+    w3, ms3, tiles, _ = run_rate(case, "masked", args.variant, mt["tile_rows"], n3, args.warmup,
+                                 args.timed_steps, launch=(mt["waves_per_block"], mt["blocks_per_cu"]))
+    return {"gcells_effective": round(case.W * case.H * n3 / w3 / 1e9, 3),
+            "kernel_ms": round(ms3, 4), "tiles_active": tiles[0], "tiles_total": tiles[1],
+            "timed_launches": args.timed_steps,
+            "tuned": {k: mt[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,6 +385,9 @@ def main():
                     help="whole runs of the 512x512 workload on the GPU and the oracle; print that line only")
     ap.add_argument("--calibrate", action="store_true",
                     help="run 1 GiB device copies (counter calibration) and exit")
+    ap.add_argument("--masked-only", action="store_true",
+                    help="run only the masked-mode leg of the headline workload (autotuned shape) and "
+                         "print its result")
     args = ap.parse_args()
     if args.calibrate:
         calibrate_copy()
@@ -405,6 +419,9 @@ def main():
     case = wet_state(config1())
     W, H = case.W, case.H
     kvar = case.kst_var is not None
+    if args.masked_only:
+        print(json.dumps({"masked": masked_leg(case, args)}), flush=True)
+        return
 
     wall, ms, _, st = run_rate(case, "dense", args.variant, args.rows, args.steps, args.warmup,
                                args.timed_steps)
@@ -449,15 +466,7 @@ def main():
         out[f"dense_{other}"] = {"gcells": round(W * H * n2 / w2 / 1e9, 3),
                                  "kernel_ms": round(ms2, 4),
                                  "roofline_frac": round(bpc * W * H / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        from fip_amd.sim import tune
-        mt = tune(case, mode="masked", variant=args.variant, steps=10)
-        n3 = max(100, args.steps // 4)
-        w3, ms3, tiles, _ = run_rate(case, "masked", args.variant, mt["tile_rows"], n3, args.warmup,
-                                     args.timed_steps, launch=(mt["waves_per_block"], mt["blocks_per_cu"]))
-        out["masked"] = {"gcells_effective": round(W * H * n3 / w3 / 1e9, 3),
-                         "kernel_ms": round(ms3, 4), "tiles_active": tiles[0],
-                         "tiles_total": tiles[1],
-                         "tuned": {k: mt[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")}}
+        out["masked"] = masked_leg(case, args)
     if not args.no_extra and not args.no_ensemble:
         out["ensemble"] = run_ensemble(args.variant, args.ensemble_steps, 50)
     if not args.no_extra and not args.no_closure:

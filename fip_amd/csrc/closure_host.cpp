@@ -393,6 +393,29 @@ int fip_closure_num_samples(fip_closure* cl, int* n) {
   return FIP_OK;
 }
 
+namespace {
+// One Adam step (beta 0.9 / 0.999, eps 1e-8) of the fp32 master weights from the gradient
+// in cl->g, then the repack of the bf16 fragments.
+int adam_update(fip_closure* cl, float lr) {
+  cl->t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)cl->t), c2 = 1.f - std::pow(b2, (float)cl->t);
+  CHIP(cl_adam_launch(cl->w, cl->m, cl->v, cl->g, CL_LAYERS, lr, b1, b2, 1e-8f, c1, c2, cl->stream));
+  return pack(cl);
+}
+}  // namespace
+
+int fip_closure_adam_step(fip_closure* cl, const float* grad, float lr) {
+  if (!cl || !grad) return fail(FIP_E_INVALID, "null argument");
+  CHIP(hipSetDevice(cl->dev));
+  CHIP(hipMemcpyAsync(cl->g, grad, sizeof(float) * CL_LAYERS * CL_LAYER_PARAMS, hipMemcpyHostToDevice,
+                      cl->stream));
+  int rc = adam_update(cl, lr);
+  if (rc) return rc;
+  CHIP(hipStreamSynchronize(cl->stream));
+  return FIP_OK;
+}
+
 int fip_closure_train(fip_closure* cl, int steps, int batch, int patch, float lr, int first, int n,
                       unsigned long long seed, float* losses) {
   if (!cl) return fail(FIP_E_INVALID, "null argument");
@@ -418,11 +441,7 @@ int fip_closure_train(fip_closure* cl, int steps, int batch, int patch, float lr
                                   ws.qp, ws.qt, ws.mask, batch, cl->stream));
     if ((rc = forward(cl, ws, nullptr, 0, cl->stream))) return rc;
     if ((rc = backward(cl, ws, kBound, cl->stream))) return rc;
-    cl->t++;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.f - std::pow(b1, (float)cl->t), c2 = 1.f - std::pow(b2, (float)cl->t);
-    CHIP(cl_adam_launch(cl->w, cl->m, cl->v, cl->g, CL_LAYERS, lr, b1, b2, 1e-8f, c1, c2, cl->stream));
-    if ((rc = pack(cl))) return rc;
+    if ((rc = adam_update(cl, lr))) return rc;
     double r, m, p;
     if ((rc = read_loss(ws, cl->stream, &r, &m, &p))) return rc;
     if (losses) losses[it] = m > 0 ? (float)(r / m) : 0.f;

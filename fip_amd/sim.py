@@ -24,7 +24,7 @@ EXPORTS = ["fip_last_error", "fip_create", "fip_destroy", "fip_bc_generate", "fi
            "fip_sync"]
 CLOSURE_EXPORTS = ["fip_restrict", "fip_closure_create", "fip_closure_destroy", "fip_closure_get_params",
                    "fip_closure_set_params", "fip_closure_add_sample", "fip_closure_num_samples",
-                   "fip_closure_train", "fip_closure_eval", "fip_closure_attach",
+                   "fip_closure_train", "fip_closure_adam_step", "fip_closure_eval", "fip_closure_attach",
                    "fip_closure_attach_ensemble", "fip_closure_detach", "fip_closure_status",
                    "fip_closure_forward", "fip_closure_backward", "fip_closure_fused_outputs",
                    "fip_closure_set_kernel", "fip_mfma_probe"]
@@ -123,6 +123,7 @@ def lib():
         L.fip_closure_num_samples.argtypes = [vp, ip]
         L.fip_closure_train.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
                                         C.c_ulonglong, fp]
+        L.fip_closure_adam_step.argtypes = [vp, fp, C.c_float]
         L.fip_closure_eval.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.fip_closure_attach.argtypes = [vp, vp, fp, C.c_float, C.c_float, C.c_double]
         L.fip_closure_attach_ensemble.argtypes = [vp, vp, fp, fp, C.c_float, C.c_double]
@@ -509,6 +510,11 @@ class Closure:
         n = C.c_int()
         _check(lib().fip_closure_num_samples(self.c, C.byref(n)))
         return n.value
+
+    def adam_step(self, grad, lr):
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        assert g.size == CL_PARAMS
+        _check(lib().fip_closure_adam_step(self.c, _f32p(g), float(lr)))
 
     def train(self, steps, batch=32, patch=128, lr=1e-3, first=0, n=None, seed=7):
         n = self.num_samples() - first if n is None else n

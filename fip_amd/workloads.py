@@ -232,6 +232,15 @@ def config1(W: int = 5667, H: int = 6500, nhills: int = 2048, duration: float = 
                 kst_var=kst, no_data=-1.0, bcs=bcs)
 
 
+def level_state(case: Case, frac: float = 0.30) -> Case:
+    """A flat water level that wets `frac` of the valid cells (h = max(0, level - bed)):
+    the late, partly flooded state of a run, as an initial state. Returns the case."""
+    valid = case.bed != np.float32(case.no_data if case.no_data is not None else -9999.0)
+    level = np.float32(np.quantile(case.bed[valid], frac))
+    case.h = np.where(valid, np.maximum(np.float32(0), level - case.bed), np.float32(0)).astype(np.float32)
+    return case
+
+
 CONFIG2_SIZES = ((256, 256), (512, 384), (1024, 1024), (1536, 1024), (2048, 1536))
 
 

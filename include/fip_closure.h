@@ -59,6 +59,11 @@ int fip_closure_num_samples(fip_closure* cl, int* n);
  * of the corrected coarse face discharges against the targets, m^4/s^2) of step s. */
 int fip_closure_train(fip_closure* cl, int steps, int batch, int patch, float lr, int first, int n,
                       unsigned long long seed, float* losses);
+/* The optimiser step fip_closure_train takes after each backward pass, from a given
+ * gradient (FIP_CL_PARAMS floats, padded layout): Adam with beta 0.9 / 0.999, eps 1e-8 on
+ * the fp32 master weights, continuing the closure's moment estimates and step count;
+ * padded parameters stay 0. For callers that form gradients themselves. */
+int fip_closure_adam_step(fip_closure* cl, const float* grad, float lr);
 /* Mean squared face-discharge error over samples [first, first + n): of the plain coarse
  * step and of the corrected one. */
 int fip_closure_eval(fip_closure* cl, int first, int n, double* mse_plain, double* mse_learned);

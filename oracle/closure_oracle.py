@@ -181,11 +181,12 @@ def live_mask():
 
 
 def adam(w, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
-    """One Adam step in float32 (as the GPU kernel); padded entries stay 0."""
+    """One Adam step in float32 (as the GPU kernel: 1 - beta is formed in binary32 from
+    the binary32 beta); padded entries stay 0."""
     f = np.float32
     live = live_mask()
-    m = (f(b1) * m + f(1 - b1) * g).astype(f)
-    v = (f(b2) * v + f(1 - b2) * g * g).astype(f)
+    m = (f(b1) * m + (f(1) - f(b1)) * g).astype(f)
+    v = (f(b2) * v + (f(1) - f(b2)) * g * g).astype(f)
     c1 = f(1) - f(b1) ** f(t)
     c2 = f(1) - f(b2) ** f(t)
     w = (w - f(lr) * (m / c1) / (np.sqrt(v / c2) + f(eps))).astype(f)
@@ -199,14 +200,29 @@ def adam(w, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
 
 def restrict(h, qx, qy, F):
     """Fine [H, W] fields -> coarse [H/F, W/F]: h = block mean; qx on the coarse east face
-    = mean of the F fine east faces at column F I + F - 1; qy on the north face likewise."""
+    = mean of the F fine east faces at column F I + F - 1; qy on the north face likewise.
+    Each sum is a sequential binary64 sum, rows of the block in order and columns in order
+    within a row, scaled by 1/F^2 (1/F) and rounded once to binary32. (A binary64 sum of
+    16 floats is exact only while their exponents lie within 25 bits of each other; a block
+    with both a nearly dry cell, say 1e-35 m, and a deep one rounds, so the order is part
+    of the definition.)"""
     H, W = h.shape
     Hc, Wc = H // F, W // F
-    hb = np.asarray(h, np.float64).reshape(Hc, F, Wc, F)
-    hc = hb.sum(axis=(1, 3)) / (F * F)
-    qxc = np.asarray(qx, np.float64)[:, F - 1::F].reshape(Hc, F, Wc).sum(axis=1) / F
-    qyc = np.asarray(qy, np.float64)[F - 1::F, :].reshape(Hc, Wc, F).sum(axis=2) / F
-    return hc.astype(np.float32), qxc.astype(np.float32), qyc.astype(np.float32)
+    h = np.asarray(h, np.float32)
+    qx = np.asarray(qx, np.float32)
+    qy = np.asarray(qy, np.float32)
+    sh = np.zeros((Hc, Wc))
+    for r in range(F):
+        for q in range(F):
+            sh += h[r::F, q::F][:Hc, :Wc].astype(np.float64)
+    sx = np.zeros((Hc, Wc))
+    sy = np.zeros((Hc, Wc))
+    for r in range(F):
+        sx += qx[r::F, F - 1::F][:Hc, :Wc].astype(np.float64)
+    for q in range(F):
+        sy += qy[F - 1::F, q::F][:Hc, :Wc].astype(np.float64)
+    inv = 1.0 / F
+    return (sh * inv * inv).astype(np.float32), (sx * inv).astype(np.float32), (sy * inv).astype(np.float32)
 
 
 def coarse_bed(bed, F):

@@ -146,6 +146,7 @@ template <class R> struct St {
   R kst;
   bool has_kvar;
   int mode, tw, tr, warp_mask, quirks, variable_dt, staggered, threads;
+  int open_pitch = 0;
   std::vector<R> bed, kv, hmax;
   std::vector<R> h[2], qx[2], qy[2];
   std::vector<float> bed_host;  // the host copy of the bed (h_sohle), float
@@ -613,8 +614,10 @@ template <class R> void apply_one_bc(St<R>& s, const BC& bc, float qvalf, float 
     } else {  // bottom
       idx = tid; stag = idx; nb = idx + pitch; q = qy; ql = qx;
     }
-    // open BC direction: +1 on the right/top side, -1 on the left/bottom side
-    const R sign = (bc.side == 1 || bc.side == 2) ? R(1) : R(-1);
+    // open BC direction: +1 on the right/top side, -1 on the left/bottom side; the
+    // reference's (float)(idx - idx_neighbor) is +-pitch on the top/bottom sides (Q8)
+    R sign = (bc.side == 1 || bc.side == 2) ? R(1) : R(-1);
+    if (s.open_pitch > 0 && bc.side >= 2) sign *= R(s.open_pitch);
     if (s.staggered) {
       const long qi = stag;
       if (bc.type == 0) {  // z
@@ -784,6 +787,7 @@ static St<R>* make_state(const orc_desc* d, const float* bed, const float* h, co
   s->tr = d->tile_rows;
   s->warp_mask = d->warp_mask;
   s->quirks = d->quirks;
+  s->open_pitch = d->open_pitch;
   s->variable_dt = d->variable_dt;
   s->staggered = d->staggered;
   s->threads = d->threads > 0 ? d->threads : 1;

@@ -45,6 +45,9 @@ typedef struct {
   int variable_dt;
   int staggered;      // 1 = solver rmg (default), 0 = solver cn
   int threads;        // OpenMP threads (0 = default)
+  int open_pitch;     // 0: the top/bottom open-BC sign is +-1 (this build); > 0: it is
+                      // +-open_pitch, the reference's (float)(idx - idx_neighbor) of a row
+                      // pitch in elements (quirk Q8, src/bc_kernel_gen.cpp:132)
 } orc_desc;
 
 typedef struct orc_state orc_state;

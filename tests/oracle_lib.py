@@ -20,7 +20,8 @@ class OrcDesc(C.Structure):
                 ("no_data", C.c_float), ("kst", C.c_float), ("has_kst_var", C.c_int),
                 ("precision", C.c_int), ("mode", C.c_int), ("tile_w", C.c_int),
                 ("tile_rows", C.c_int), ("warp_mask", C.c_int), ("quirks", C.c_int),
-                ("variable_dt", C.c_int), ("staggered", C.c_int), ("threads", C.c_int)]
+                ("variable_dt", C.c_int), ("staggered", C.c_int), ("threads", C.c_int),
+                ("open_pitch", C.c_int)]
 
 
 _lib = None
@@ -79,14 +80,14 @@ FIELDS = {"h": 0, "qx": 1, "qy": 2, "hmax": 3, "bed": 4, "kst": 5}
 class Oracle:
     def __init__(self, case: Case, mode: int = DENSE, precision: int = 32, tile_w: int = 60,
                  tile_rows: int = 64, warp_mask: int = 1, quirks: int = 0, threads: int = 0,
-                 staggered: int = 1, with_bcs: bool = True):
+                 staggered: int = 1, with_bcs: bool = True, open_pitch: int = 0):
         L = lib()
         self.case = case
         self.W, self.H = case.W, case.H
         d = OrcDesc(case.W, case.H, case.dx, case.dt,
                     case.no_data if case.no_data is not None else -9999.0, case.kst,
                     1 if case.kst_var is not None else 0, precision, mode, tile_w, tile_rows,
-                    warp_mask, quirks, 1 if case.variable_dt else 0, staggered, threads)
+                    warp_mask, quirks, 1 if case.variable_dt else 0, staggered, threads, open_pitch)
         self._keep = []
 
         def arr(a):

@@ -34,6 +34,23 @@ def test_restrict_conserves_volume_and_discharge():
             assert abs(qyc[J, I] * F - fine) < 1e-5
 
 
+def test_restrict_rounds_a_wide_block_in_sequential_order():
+    # a block holding nearly dry cells (1e-35 m) beside deep ones: the binary64 sum is not
+    # exact, and the sequential order of the definition gives the correctly rounded mean
+    # here where summing row sums first does not
+    from fractions import Fraction
+    v = np.array([0.0, 5.430896994236511e-17, 0.6958233714103699, 0.7572036385536194,
+                  3.762364820236067e-35, 5.900398158651158e-16, 0.6636788845062256, 0.6971497535705566,
+                  1.932195512948111e-33, 0.8240683674812317, 0.8419486880302429, 0.7811002135276794,
+                  4.914276678898646e-32, 0.7862984538078308, 0.8122485280036926, 0.875024139881134],
+                 np.float32).reshape(4, 4)
+    hc, _, _ = co.restrict(v, v, v, 4)
+    exact = sum(Fraction(float(x)) for x in v.ravel()) / 16
+    assert hc[0, 0] == np.float32(float(exact))
+    rows_first = np.float32(v.astype(np.float64).sum(axis=1).sum() / 16)
+    assert rows_first != hc[0, 0]
+
+
 def test_restrict_brute_force_and_constants():
     F = 4
     h = np.arange(8 * 12, dtype=np.float32).reshape(8, 12) * np.float32(0.25)

@@ -292,3 +292,59 @@ def test_closure_ensemble_deterministic():
         cl.close()
     for a, b in zip(*runs):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_adam_step_matches_oracle():
+    """The optimiser step of training (cl_adam) equals the oracle's binary32 Adam bit for
+    bit over consecutive steps, moment estimates and bias corrections included."""
+    from fip_amd.sim import Closure
+    rng = np.random.default_rng(21)
+    cl = Closure(16, 16, 0, seed=4)
+    w = rand_params(rng, 0.3)
+    cl.params = w
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    for t in range(1, 4):
+        g = np.where(co.live_mask(), rng.normal(0, 1e-2, co.NPARAMS), 0).astype(np.float32)
+        cl.adam_step(g, 1e-3)
+        w, m, v = co.adam(w, m, v, g, t, 1e-3)
+        got = cl.params
+        bad = np.count_nonzero(got.view(np.uint32) != w.view(np.uint32))
+        assert bad == 0, (t, bad, np.abs(got - w).max())
+    cl.close()
+
+
+def test_mass_error_fallback_gives_the_plain_step():
+    """A mass-balance limit below any error (-1: the relative error is never negative)
+    switches the domain to the plain step at the first check (the second attached step:
+    the first one records the volume); from there on the domain equals a run that took the
+    same two corrected steps and then ran without the closure, bit for bit."""
+    from fip_amd.sim import Closure, Sim
+    fine, coarse, bstd, zref = fine_coarse_case(256, 256, 1.0, 4, scenario=4, wet=0.5)
+    rng = np.random.default_rng(3)
+    p = rand_params(rng, 0.3)
+    runs = []
+    for limit in (-1.0, 1e30):
+        cl = Closure(64, 64, 0, seed=9)
+        cl.params = p
+        s = Sim(coarse, mode="dense")
+        s.set_run(1e9)
+        s.simulate(20)
+        cl.attach(s, bstd, zref, sat=50.0, mass_limit=limit)
+        s.simulate(1)
+        assert cl.status(s)["fallback"] == 0
+        s.simulate(1)
+        st = cl.status(s)
+        if limit > 0:
+            assert st["fallback"] == 0
+            Closure.detach(s)  # the reference: plain steps from the same corrected state
+        else:
+            assert st["fallback"] == 1, st
+        s.simulate(60)
+        if limit < 0:
+            assert cl.status(s)["fallback"] == 1
+        runs.append({(k, side): s.field(k, side) for k in ("h", "qx", "qy") for side in (0, 1)})
+        s.close()
+        cl.close()
+    for key in runs[0]:
+        assert np.array_equal(runs[0][key].view(np.uint32), runs[1][key].view(np.uint32)), key
