@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -274,6 +275,35 @@ int fip_restrict(fip_sim* fine, fip_sim* coarse, int F) {
   return FIP_OK;
 }
 
+// Scratch of fip_closure_forward_noalloc: four bf16 images of 32 channels for up to
+// FIP_CL_NOALLOC_MAX_PIXELS pixels (64 MiB): the network input, one pre-activation image
+// that the three hidden layers overwrite in turn (the forward pass never reads it back),
+// and two activation images used alternately. Every pixel and channel of an image is
+// written before it is read, so nothing needs clearing between calls. Each device holds
+// its own copy; its address is looked up once per device, at fip_closure_create.
+namespace {
+constexpr size_t kNoAllocImage = (size_t)FIP_CL_NOALLOC_MAX_PIXELS * CL_C;
+constexpr int kNoAllocDevices = 64;
+void* g_scratch_ptr[kNoAllocDevices] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+static __device__ __align__(256) u16 scratch[4 * kNoAllocImage];
+
+namespace {
+// With `device` current: the device address of its copy of `scratch`.
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return fail(FIP_E_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return FIP_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(FIP_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return FIP_OK;
+}
+}  // namespace
+
 int fip_closure_create(int device, unsigned long long seed, int Wc, int Hc, int max_samples, fip_closure** out) {
   if (!out) return fail(FIP_E_INVALID, "null argument");
   *out = nullptr;
@@ -291,7 +321,9 @@ int fip_closure_create(int device, unsigned long long seed, int Wc, int Hc, int 
   if (hipStreamCreateWithFlags(&cl->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipStreamCreate failed"));
   const size_t np = (size_t)CL_LAYERS * CL_LAYER_PARAMS;
-  int rc = dalloc(&cl->w, np);
+  int rc = noalloc_setup(device);
+  if (rc) return cleanup(rc);
+  rc = dalloc(&cl->w, np);
   if (!rc) rc = dalloc(&cl->m, np);
   if (!rc) rc = dalloc(&cl->v, np);
   if (!rc) rc = dalloc(&cl->g, np);
@@ -609,6 +641,33 @@ int fip_closure_forward(fip_closure* cl, const float* feat, int N, int H, int W,
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("forward: ") + hipGetErrorString(e));
   if (rc) return rc;
   CHIP(hipMemcpy(out, ws.o, npix * 2 * sizeof(float), hipMemcpyDeviceToHost));
+  return FIP_OK;
+}
+
+int fip_closure_forward_noalloc(fip_closure* cl, const float* feat, int N, int H, int W, float* out) {
+  if (!cl || !feat || !out) return fail(FIP_E_INVALID, "null argument");
+  if (N < 1 || H < 1 || W < 1) return fail(FIP_E_INVALID, "empty image batch");
+  if (N > 65535) return fail(FIP_E_INVALID, "at most 65535 images per call");
+  const long npix = (long)N * H * W;
+  if (npix > FIP_CL_NOALLOC_MAX_PIXELS)
+    return fail(FIP_E_INVALID, "N*H*W = " + std::to_string(npix) + " pixels exceeds the scratch of " +
+                                   std::to_string(FIP_CL_NOALLOC_MAX_PIXELS) + " pixels");
+  CHIP(hipSetDevice(cl->dev));
+  u16* base = (u16*)g_scratch_ptr[cl->dev];
+  if (!base) return fail(FIP_E_STATE, "scratch not set up on this device");
+  u16* x0 = base;
+  u16* z = base + kNoAllocImage;
+  u16* a[2] = {base + 2 * kNoAllocImage, base + 3 * kNoAllocImage};
+  CHIP(cl_feat_from_float_launch(feat, npix, x0, cl->stream));
+  const u16* in = x0;
+  for (int l = 0; l < 3; l++) {
+    CHIP(cl_launch_conv(EPI_GELU, in, cl->wf + (size_t)l * CL_PACKED, bias_of(cl, l), nullptr, z, a[l & 1],
+                        nullptr, N, H, W, nullptr, 0, cl->stream));
+    in = a[l & 1];
+  }
+  CHIP(cl_launch_conv(EPI_OUT, in, cl->wf + 3 * (size_t)CL_PACKED, bias_of(cl, 3), nullptr, nullptr, nullptr, out,
+                      N, H, W, nullptr, 0, cl->stream));
+  CHIP(hipStreamSynchronize(cl->stream));
   return FIP_OK;
 }
 

@@ -26,7 +26,7 @@ CLOSURE_EXPORTS = ["fip_restrict", "fip_closure_create", "fip_closure_destroy", 
                    "fip_closure_set_params", "fip_closure_add_sample", "fip_closure_num_samples",
                    "fip_closure_train", "fip_closure_adam_step", "fip_closure_eval", "fip_closure_attach",
                    "fip_closure_attach_ensemble", "fip_closure_detach", "fip_closure_status",
-                   "fip_closure_forward", "fip_closure_backward", "fip_closure_fused_outputs",
+                   "fip_closure_forward", "fip_closure_forward_noalloc", "fip_closure_backward", "fip_closure_fused_outputs",
                    "fip_closure_set_kernel", "fip_mfma_probe"]
 CL_PARAMS = 4 * (9 * 32 * 32 + 32)
 ENS_EXPORTS = ["fip_ens_create", "fip_ens_destroy", "fip_ens_set_bcs", "fip_ens_set_run",
@@ -130,6 +130,7 @@ def lib():
         L.fip_closure_detach.argtypes = [vp]
         L.fip_closure_status.argtypes = [vp, vp, vp, C.c_int, ip, C.POINTER(C.c_uint), dp]
         L.fip_closure_forward.argtypes = [vp, fp, C.c_int, C.c_int, C.c_int, fp]
+        L.fip_closure_forward_noalloc.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, vp]
         L.fip_closure_backward.argtypes = [vp, fp, C.c_int, C.c_int, C.c_int, fp, fp,
                                            C.POINTER(C.c_ubyte), C.c_float, fp, dp]
         L.fip_closure_fused_outputs.argtypes = [vp, vp, fp, C.c_float, fp]
@@ -555,6 +556,11 @@ class Closure:
         out = np.zeros((N, H, W, 2), np.float32)
         _check(lib().fip_closure_forward(self.c, _f32p(f), N, H, W, _f32p(out)))
         return out
+
+    def forward_noalloc(self, feat_dev, N, H, W, out_dev):
+        """fip_closure_forward_noalloc on device pointers (integers): feat [N][H][W][6] and
+        out [N][H][W][2] fp32 in device memory."""
+        _check(lib().fip_closure_forward_noalloc(self.c, C.c_void_p(feat_dev), N, H, W, C.c_void_p(out_dev)))
 
     def set_kernel(self, form: int):
         """Launch form of the fused inference kernel: 1 persistent (default), 0 per tile."""

@@ -90,6 +90,18 @@ int fip_closure_forward(fip_closure* cl, const float* feat, int N, int H, int W,
 int fip_closure_backward(fip_closure* cl, const float* feat, int N, int H, int W, const float* qp,
                          const float* qt, const unsigned char* mask, float bound, float* grad,
                          double* loss);
+/* fip_closure_forward without allocating: feat and out are DEVICE pointers on the
+ * closure's device (feat [N][H][W][6] fp32, ready before the call; out [N][H][W][2] fp32),
+ * and the activations live in a static device array of the library (64 MiB per device,
+ * held until the process ends) instead of buffers it allocates. N*H*W must not exceed
+ * FIP_CL_NOALLOC_MAX_PIXELS (a 512 x 512 coarse grid) and N must not exceed 65535;
+ * larger batches are rejected with FIP_E_INVALID. Runs on the closure's stream and
+ * returns when out is written. All calls on one device share that array: calls that
+ * overlap in time on one device race. No counterpart in the reference: for callers
+ * that cannot allocate device memory (tight serving loops, code that must not call
+ * hipMalloc). Outputs equal fip_closure_forward's bit for bit. */
+#define FIP_CL_NOALLOC_MAX_PIXELS (512 * 512)
+int fip_closure_forward_noalloc(fip_closure* cl, const float* feat, int N, int H, int W, float* out);
 /* The raw outputs [H][W][2] of the fused inference kernel on the coarse simulation's
  * current state (no correction applied): checks it against fip_closure_forward. */
 int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed_std, float zref,
