@@ -909,6 +909,8 @@ float orc_sum_water_depths(orc_state* s, int i, float z) {
   DISPATCH(r = sum_water_depths(S, S.bcs[i], z));
   return r;
 }
+static long long bad_slot_counter_ = 0;
+
 float orc_interp(const float* t, const float* v, int n, float time) {
   Series se;
   se.t.assign(t, t + n);
@@ -916,6 +918,31 @@ float orc_interp(const float* t, const float* v, int n, float time) {
   return se.interp(time);
 }
 float orc_pow233(float x) { return fip_pow233(x); }
+
+// Every binary32 x with lo <= x <= hi against x^2.33f in long double (64-bit significand)
+// rounded to binary32: the inputs counted and those where fip_pow233 differs, with up to
+// `cap` of them written to bad_x.
+void orc_pow233_sweep(float lo, float hi, long long* n, long long* nbad, float* bad_x, int cap) {
+  const uint32_t a = fip_f2u(lo), b = fip_f2u(hi);
+  long long cnt = 0, bad = 0;
+  const long double p = (long double)2.33f;
+  bad_slot_counter_ = 0;
+#pragma omp parallel for schedule(static) reduction(+ : cnt, bad)
+  for (long long u = a; u <= (long long)b; u++) {
+    const float x = fip_u2f((uint32_t)u);
+    const float want = (float)powl((long double)x, p);
+    cnt++;
+    if (fip_pow233(x) != want) {
+      long long k;
+#pragma omp atomic capture
+      k = bad_slot_counter_++;
+      if (k < cap) bad_x[k] = x;
+      bad++;
+    }
+  }
+  *n = cnt;
+  *nbad = bad;
+}
 
 void orc_get_time(orc_state* s, double* t, double* dt, int* it) {
   DISPATCH({

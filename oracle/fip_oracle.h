@@ -74,6 +74,8 @@ void orc_bc_params(orc_state* s, int i, float* qval, float* zval);
 float orc_sum_water_depths(orc_state* s, int i, float z);
 float orc_interp(const float* t, const float* v, int n, float time);
 float orc_pow233(float x);
+// Exhaustive check of fip_pow233 on [lo, hi] against long double pow (see fip_oracle.cpp).
+void orc_pow233_sweep(float lo, float hi, long long* n, long long* nbad, float* bad_x, int cap);
 
 void orc_get_time(orc_state* s, double* t, double* dt, int* iteration);
 void orc_set_dt(orc_state* s, double dt);

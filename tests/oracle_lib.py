@@ -50,6 +50,8 @@ def lib():
         L.orc_interp.restype = C.c_float
         L.orc_pow233.argtypes = [C.c_float]
         L.orc_pow233.restype = C.c_float
+        L.orc_pow233_sweep.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                       C.POINTER(C.c_float), C.c_int]
         L.orc_get_time.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_int)]
         L.orc_set_dt.argtypes = [C.c_void_p, C.c_double]
@@ -192,6 +194,14 @@ class Oracle:
 
 def pow233(x: float) -> float:
     return lib().orc_pow233(x)
+
+
+def pow233_sweep(lo, hi, cap=64):
+    """Every binary32 x in [lo, hi] against long double pow: (inputs, misrounded, their x)."""
+    n, nb = C.c_longlong(), C.c_longlong()
+    bad = np.zeros(cap, np.float32)
+    lib().orc_pow233_sweep(lo, hi, C.byref(n), C.byref(nb), _fp(bad), cap)
+    return n.value, nb.value, bad[:min(cap, nb.value)]
 
 
 def interp(t, v, time):

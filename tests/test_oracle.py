@@ -1,6 +1,7 @@
 """Pins the CPU oracle: against the reference's literal kernel structure, closed forms,
 invariants and an independent binary64 evaluation. CPU only."""
 import math
+import os
 import struct
 
 import numpy as np
@@ -39,6 +40,62 @@ def test_pow233_correctly_rounded_on_sweep():
         ulp = np.longdouble(np.spacing(np.float32(got)))
         worst = max(worst, float(abs(np.longdouble(got) - ref) / ulp))
     assert worst <= 0.5 + 1e-6, worst
+
+
+def test_pow233_exhaustive_on_the_friction_domain():
+    # every binary32 x in [0.01, 100] (the reciprocal of a face depth of 0.01 m to 100 m):
+    # correctly rounded except where x^2.33f lies within 2^-48 relative of a rounding
+    # midpoint, checked for each miss with 60-digit arithmetic
+    from decimal import Decimal, getcontext
+    from oracle_lib import pow233_sweep
+    getcontext().prec = 60
+    n, nbad, bad = pow233_sweep(0.01, 100.0)
+    assert n > 1.1e8
+    assert nbad <= 4, nbad
+    p = Decimal(float(np.float32(2.33)))
+    for x in bad:
+        exact = (Decimal(float(x)).ln() * p).exp()
+        got = np.float32(pow233(float(x)))
+        other = np.nextafter(got, np.float32(np.inf) if Decimal(float(got)) < exact else np.float32(0))
+        mid = (Decimal(float(got)) + Decimal(float(other))) / 2
+        assert abs(exact - mid) / exact < Decimal(2) ** -48, (x, exact, got)
+
+
+def test_pow233_tables_are_rounded_values():
+    # c_i = 128 / (129 + 2 i) to 28 significant bits (nearest), L_i = -ln c_i and
+    # T_j = 2^(j / 64) correctly rounded to binary64, as the header states
+    import re
+    from decimal import Decimal, getcontext
+    from fractions import Fraction
+    getcontext().prec = 60
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            "fip_amd", "csrc", "fip_detmath.h")).read()
+
+    def table(name):
+        body = re.search(name + r"\[64\] = \{(.*?)\};", src, re.S).group(1)
+        return [float.fromhex(v) for v in re.findall(r"0x[0-9a-fp.+-]+", body)]
+
+    def rn(x, bits):
+        f = Fraction(x)
+        e = 0
+        while f >= 2:
+            f /= 2
+            e += 1
+        while f < 1:
+            f *= 2
+            e -= 1
+        s = f * (1 << (bits - 1))
+        k = int(s)
+        if s - k > Fraction(1, 2) or (s - k == Fraction(1, 2) and k % 2):
+            k += 1
+        return float(Fraction(k, 1 << (bits - 1)) * Fraction(2) ** e)
+
+    c, lt, tt = table("kFipPowC"), table("kFipPowL"), table("kFipPowT")
+    assert len(c) == len(lt) == len(tt) == 64
+    for i in range(64):
+        assert c[i] == rn(Fraction(128, 129 + 2 * i), 28), i
+        assert lt[i] == rn(-(Decimal(c[i]).ln()), 53), i
+        assert tt[i] == rn(Decimal(2) ** (Decimal(i) / 64), 53), i
 
 
 def test_pow233_special_values():
