@@ -17,10 +17,10 @@
 #include "fip_embed.h"  // generated: the device headers as strings for hipRTC
 #include "fip_internal.h"
 
-hipError_t fip_launch_stencil_exact(const fip_grid& g, int par, int masked, int blocks, int wpb,
-                                    int span, hipStream_t st);
-hipError_t fip_launch_stencil_fast(const fip_grid& g, int par, int masked, int blocks, int wpb,
-                                   int span, hipStream_t st);
+hipError_t fip_launch_stencil_exact(const fip_grid& g, int par, int masked, int blocks, int wpb, int span,
+                                    int flood, hipStream_t st);
+hipError_t fip_launch_stencil_fast(const fip_grid& g, int par, int masked, int blocks, int wpb, int span,
+                                   int flood, hipStream_t st);
 hipError_t fip_stencil_dense_waves_per_cu_exact(int kvar, int wpb, int* waves);
 hipError_t fip_stencil_dense_waves_per_cu_fast(int kvar, int wpb, int* waves);
 hipError_t fip_launch_misc(int which, const fip_grid& g, int arg, int blocks, hipStream_t st);
@@ -99,10 +99,12 @@ struct fip_sim {
   // kernel, which tests iteration % 100 itself), most recently built last
   struct Chunk {
     int par, steps;
+    bool flood_first;
     hipGraphExec_t exec;
   };
   std::vector<Chunk> graphs;
   bool graph_failed = false;
+  long long flood_done_iter = -1;  // dense mode: the flood-plain iteration last taken by the separate pass
   fip_internal::StepHook post_hook;  // extra work after each step of the device-resident loop
   // stats / timing
   bool timing = false;
@@ -142,14 +144,15 @@ int plan_dense(fip_sim* s) {
 // march in step; 16 measured 0.5 % faster than 8 at 5667 x 6500), 4 otherwise.
 int default_wpb(const fip_sim* s) { return !s->masked && s->fast ? 16 : 4; }
 
-int launch_stencil(fip_sim* s, int par) {
+// flood (dense mode): the step also takes the flood plains of the state it reads.
+int launch_stencil(fip_sim* s, int par, int flood = 0) {
   const int kind = s->masked ? (s->bpc > 0 ? 2 : 1) : 0;
   const int blocks = kind == 2 ? s->stencil_blocks
                      : kind == 1 ? (s->ntiles + s->wpb - 1) / s->wpb : s->dense_blocks;
   hipError_t e;
   if (s->timing) HIPTRY(hipEventRecord(s->ev0, s->stream));
-  e = s->fast ? fip_launch_stencil_fast(s->g, par, kind, blocks, s->wpb, s->dense_span, s->stream)
-              : fip_launch_stencil_exact(s->g, par, kind, blocks, s->wpb, s->dense_span, s->stream);
+  e = s->fast ? fip_launch_stencil_fast(s->g, par, kind, blocks, s->wpb, s->dense_span, flood, s->stream)
+              : fip_launch_stencil_exact(s->g, par, kind, blocks, s->wpb, s->dense_span, flood, s->stream);
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
   if (s->timing) {
     HIPTRY(hipEventRecord(s->ev1, s->stream));
@@ -509,8 +512,8 @@ void destroy_graphs(fip_sim* s) {
   s->graphs.clear();
 }
 
-int one_step(fip_sim* s, int par) {
-  int rc = launch_stencil(s, par);
+int one_step(fip_sim* s, int par, int flood = 0) {
+  int rc = launch_stencil(s, par, flood);
   if (rc) return rc;
   rc = launch_bc(s, BC_POST, par);
   if (rc) return rc;
@@ -518,19 +521,28 @@ int one_step(fip_sim* s, int par) {
 }
 
 // Captured chunks kept per simulation: a run uses at most three lengths (the steps up to
-// the next multiple of kStepsPerGraph, whole chunks, the rest), each at one or two parities.
+// the next multiple of kStepsPerGraph, whole chunks, the rest), each at one or two parities
+// and with or without the flood plains in its first step.
 constexpr size_t kMaxGraphs = 8;
 
-// The graph of `steps` steps (1..kStepsPerGraph) starting at parity par, plus the
-// flood-plain kernel; captured and instantiated on first use. nullptr when capture failed.
-hipGraphExec_t chunk_graph(fip_sim* s, int par, int steps) {
+// Flood plains (src/main.cpp:125-128: after the step that reaches iteration % 100 == 0).
+// In dense mode the next stencil step, which reads that state, takes them for the cells it
+// owns (flood_first); in masked mode, and when a run returns at such an iteration, the
+// separate pass does (it is idempotent: max(hmax, h) of the same state).
+bool fused_flood(const fip_sim* s) { return !s->masked; }
+
+// The graph of `steps` steps (1..kStepsPerGraph) starting at parity par; flood_first: the
+// first step takes the flood plains. In masked mode it ends with the flood-plain pass,
+// which tests iteration % 100 itself. Captured and instantiated on first use; nullptr
+// when capture failed.
+hipGraphExec_t chunk_graph(fip_sim* s, int par, int steps, bool flood_first) {
   for (auto& c : s->graphs)
-    if (c.par == par && c.steps == steps) return c.exec;
+    if (c.par == par && c.steps == steps && c.flood_first == flood_first) return c.exec;
   hipGraph_t graph;
   if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
   int rc = FIP_OK;
-  for (int k = 0; k < steps && rc == FIP_OK; k++) rc = one_step(s, par ^ (k & 1));
-  if (rc == FIP_OK) rc = launch_misc(s, 3, 1, 0);
+  for (int k = 0; k < steps && rc == FIP_OK; k++) rc = one_step(s, par ^ (k & 1), k == 0 && flood_first);
+  if (rc == FIP_OK && !fused_flood(s)) rc = launch_misc(s, 3, 1, 0);
   const hipError_t e = hipStreamEndCapture(s->stream, &graph);
   if (rc != FIP_OK || e != hipSuccess) return nullptr;
   hipGraphExec_t exec = nullptr;
@@ -541,29 +553,36 @@ hipGraphExec_t chunk_graph(fip_sim* s, int par, int steps) {
     hipGraphExecDestroy(s->graphs.front().exec);
     s->graphs.erase(s->graphs.begin());
   }
-  s->graphs.push_back({par, steps, exec});
+  s->graphs.push_back({par, steps, flood_first, exec});
   return exec;
+}
+
+// Whether the chunk that starts at iteration it0 takes the flood plains in its first step.
+bool flood_first_at(const fip_sim* s, long long it0) {
+  return fused_flood(s) && it0 > 0 && it0 % kStepsPerGraph == 0 && s->flood_done_iter != it0;
 }
 
 // The chunks fip_simulate cuts `nsteps` steps from iteration it0 into: up to the next
 // multiple of kStepsPerGraph (the flood-plain cadence), then whole chunks. Calls
-// f(parity, steps) for each distinct chunk in order.
+// f(parity, steps, flood_first) for each distinct chunk in order.
 template <class F>
-void for_each_chunk(int it0, long long nsteps, F f) {
+void for_each_chunk(const fip_sim* s, int it0, long long nsteps, F f) {
   long long it = it0, left = nsteps;
   int seen = 0;
-  int last_par = -1, last_k = -1;
-  while (left > 0 && seen < 6) {
+  int last_par = -1, last_k = -1, last_fl = -1;
+  while (left > 0 && seen < 8) {
     const long long k = std::min<long long>(kStepsPerGraph - it % kStepsPerGraph, left);
     const int par = (int)(it & 1);
-    if (par != last_par || k != last_k) {
-      f(par, (int)k);
+    const int fl = flood_first_at(s, it) ? 1 : 0;
+    if (par != last_par || k != last_k || fl != last_fl) {
+      f(par, (int)k, fl != 0);
       seen++;
       last_par = par;
       last_k = (int)k;
+      last_fl = fl;
     }
     // whole chunks keep their parity (kStepsPerGraph is even): skip to the last one
-    if (k == kStepsPerGraph && left >= 2 * kStepsPerGraph) {
+    if (k == kStepsPerGraph && left >= 2 * kStepsPerGraph && it >= kStepsPerGraph) {
       const long long whole = left / kStepsPerGraph;
       it += (whole - 1) * kStepsPerGraph;
       left -= (whole - 1) * kStepsPerGraph;
@@ -990,9 +1009,10 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
     const int it0 = s->host_iter;
     const long long to_flood = kStepsPerGraph - (it0 % kStepsPerGraph);
     const long long k = std::min<long long>(to_flood, nsteps - done);
+    const bool flood_first = flood_first_at(s, it0);
     hipGraphExec_t gx = nullptr;
     if (!s->timing && !s->graph_failed) {
-      gx = chunk_graph(s, it0 & 1, (int)k);
+      gx = chunk_graph(s, it0 & 1, (int)k, flood_first);
       if (!gx) {
         s->graph_failed = true;
         hipGetLastError();
@@ -1003,8 +1023,8 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
       s->stats.stencil_launches += k;
     } else {
       for (long long i = 0; i < k; i++)
-        if ((rc = one_step(s, (it0 + (int)i) & 1))) return rc;
-      if ((rc = launch_misc(s, 3, 1, 0))) return rc;
+        if ((rc = one_step(s, (it0 + (int)i) & 1, i == 0 && flood_first))) return rc;
+      if (!fused_flood(s) && (rc = launch_misc(s, 3, 1, 0))) return rc;
     }
     int it1;
     if ((rc = ctl_read(s, CTL_OFF(iteration), &it1))) return rc;
@@ -1031,6 +1051,12 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
     }
   }
   if (steps_done) *steps_done = done;
+  // the run returns on a flood-plain iteration: take them now (the next step would)
+  if (fused_flood(s) && s->host_iter > 0 && s->host_iter % kStepsPerGraph == 0 &&
+      s->flood_done_iter != s->host_iter) {
+    if ((rc = launch_misc(s, 3, 1, 0))) return rc;
+    s->flood_done_iter = s->host_iter;
+  }
   if (s->masked) {
     unsigned cnt[2];
     if ((rc = ctl_read(s, CTL_OFF(list_count), cnt, 2))) return rc;
@@ -1049,8 +1075,8 @@ int fip_prepare(fip_sim* s, long long nsteps) {
   HIPTRY(hipSetDevice(s->dev));
   if (s->timing || s->graph_failed) return FIP_OK;
   bool ok = true;
-  for_each_chunk(s->host_iter, nsteps, [&](int par, int k) {
-    if (ok && !chunk_graph(s, par, k)) ok = false;
+  for_each_chunk(s, s->host_iter, nsteps, [&](int par, int k, bool fl) {
+    if (ok && !chunk_graph(s, par, k, fl)) ok = false;
   });
   if (!ok) {
     s->graph_failed = true;

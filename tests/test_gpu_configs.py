@@ -49,20 +49,25 @@ def log(msg):
 
 
 def test_config1_dense_exact_bit_exact(cfg1):
+    """105 steps: past the flood plains of iteration 100, which dense mode takes inside the
+    stencil step that follows."""
     from fip_amd.sim import Sim
     case = cfg1
-    log("config1 5667x6500 dense exact: 20 steps on the GPU")
+    steps = 105
+    log(f"config1 5667x6500 dense exact: {steps} steps on the GPU")
     g = Sim(case, mode="dense", variant="exact")
     g.set_run(case.duration)
-    ev, done = g.simulate(20)
-    assert done == 20
-    log(f"config1 dense: 20 oracle steps on {THREADS} threads")
+    ev, done = g.simulate(steps)
+    assert done == steps
+    log(f"config1 dense: {steps} oracle steps on {THREADS} threads")
     o = Oracle(case, DENSE, threads=THREADS)
-    o.step(20)
+    o.step(steps)
     o.control()  # the device loop has already taken the next step's dt (end of its post kernel)
     assert_state_equal(g, o, "config1 dense exact")
     assert_time_equal(g, o)
     assert np.float32(g.cfl()) == np.float32(o.cfl_min())
+    assert same_bits(g.field("hmax"), o.field("hmax"))
+    assert not np.array_equal(o.field("hmax"), case.h)  # the flood plains changed it
     g.close()
 
 
