@@ -339,13 +339,17 @@ def masked_leg(case, args):
     """Masked mode on the headline workload with the autotuner's shape: effective rate
     (W*H per step over wall time), stencil time, active tiles."""
     from fip_amd.sim import tune
-    mt = tune(case, mode="masked", variant=args.variant, steps=10)
+    if args.masked_shape:
+        r, w, b = (int(v) for v in args.masked_shape.split(","))
+        mt = {"tile_rows": r, "waves_per_block": w, "blocks_per_cu": b}
+    else:
+        mt = tune(case, mode="masked", variant=args.variant, steps=10)
     n3 = max(100, args.steps // 4)
     w3, ms3, tiles, _ = run_rate(case, "masked", args.variant, mt["tile_rows"], n3, args.warmup,
                                  args.timed_steps, launch=(mt["waves_per_block"], mt["blocks_per_cu"]))
     return {"gcells_effective": round(case.W * case.H * n3 / w3 / 1e9, 3),
             "kernel_ms": round(ms3, 4), "tiles_active": tiles[0], "tiles_total": tiles[1],
-            "timed_launches": args.timed_steps,
+            "timed_launches": args.timed_steps, "shape_from": "given" if args.masked_shape else "autotuner",
             "tuned": {k: mt[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")}}
 
 
@@ -388,6 +392,9 @@ def main():
     ap.add_argument("--masked-only", action="store_true",
                     help="run only the masked-mode leg of the headline workload (autotuned shape) and "
                          "print its result")
+    ap.add_argument("--masked-shape", default=None,
+                    help="masked leg: tile rows, waves per work-group, work-groups per CU (e.g. 16,2,4, "
+                         "the fip CLI's default) instead of the autotuner's pick")
     args = ap.parse_args()
     if args.calibrate:
         calibrate_copy()
@@ -447,7 +454,9 @@ def main():
         "config": {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q BC, "
                                "right open, 30% wet, dense mode",
                    "W": W, "H": H, "mode": "dense", "variant": args.variant,
-                   "tile": f"60 columns x {st['dense_span']}-row spans, 16 wavefronts in step",
+                   "tile": f"60 columns x {st['dense_span']}-row spans, " +
+                           ("16 wavefronts per work-group in step" if args.variant == "fast"
+                            else "1 wavefront per work-group"),
                    "kst": f"{st['kst_classes']} classes, 1 B per cell" if st["kst_classes"] else "f32",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -10,6 +10,10 @@
 //   march/8   the same with the 8 wavefronts of a work-group on adjacent tile columns,
 //             stepping through the rows in step (a barrier every second row), as the fast
 //             dense kernel does;
+//   march/lds the march with LDS-staged rows instead of per-lane loads: the work-group's
+//             threads load each row of its 8 tile columns (484 columns with the halo)
+//             once into LDS, and every lane reads its column and both neighbours from
+//             LDS (two barriers per row) in place of the DPP lane shifts;
 //   flat      the same byte counts as one flat stream: 16-byte accesses per lane
 //             (4 bytes per lane for the class field), grid-stride;
 //   flat/nt   the flat stream with non-temporal (streaming) stores;
@@ -21,7 +25,14 @@
 //   write     the flat stream's writes alone (3 floats per cell);
 //   copy4     four fields copied at once (4 reads + 4 writes: eight streams at 1:1);
 //   stagger   the flat stream with the eight fields in one allocation, field k starting
-//             k * (field size + s) bytes in, for a stagger s.
+//             k * (field size + s) bytes in, for a stagger s;
+//   seg/soa   one wavefront per 64-column row segment, 4 bytes per lane and field, the
+//             fields in separate arrays: eight streams (h, qx, qy, bed, class read;
+//             h, qx, qy written);
+//   seg/aosoa the same accesses with h, qx, qy of a segment contiguous (64 h, then 64 qx,
+//             then 64 qy: 768 B), bed and the class in their own arrays: four streams
+//             (the h|qx|qy array and bed and class read, an h|qx|qy array written).
+//             The segments cover the 5696-column pitch (0.5 % more cells than the grid).
 // Every access stays inside its allocation: rows are below H, columns are clamped to
 // [0, W - 1], flat indices are below W * H / 4, and the staggered fields end inside the
 // allocation (checked on the host).
@@ -61,6 +72,46 @@ __global__ __launch_bounds__(512) void march(Fields f, int span, int tiles_x, in
     for (int k = 0; k < kIn; k++) s += f.in[k][o];
     acc = acc * 0.5f + s;
     if (own && r >= gfirst) {
+#pragma unroll
+      for (int k = 0; k < kOut; k++) f.out[k][o] = acc + (float)k;
+    }
+  }
+}
+
+// LDS-staged march: 8 wavefronts of 60-column tiles, rows staged through LDS.
+__global__ __launch_bounds__(512) void march_lds(Fields f, int span, int groups) {
+  constexpr int kCols = 8 * 60 + 4;
+  __shared__ float sh[kIn][kCols];
+  __shared__ float scl[kCols];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int sr = blockIdx.x / groups;
+  const int c0 = (blockIdx.x % groups) * 8 * 60 - 2;  // column of LDS slot 0
+  const int my = wave * 60 + lane;                     // this lane's slot, 0 .. kCols - 1
+  const int c = c0 + my;
+  const bool own = lane >= 2 && lane < 62 && c >= 0 && c < kW;
+  const int cst = c < 0 ? 0 : (c >= kW ? kW - 1 : c);
+  int cl = c0 + t;
+  cl = cl < 0 ? 0 : (cl >= kW ? kW - 1 : cl);
+  const int lw = my > 0 ? my - 1 : 0, le = my < kCols - 1 ? my + 1 : kCols - 1;
+  const int gfirst = sr * span;
+  const int r0 = gfirst >= 2 ? gfirst - 2 : 0;
+  const int r1 = gfirst + span < kH ? gfirst + span : kH;
+  float acc = 0.f;
+  for (int r = r0; r < r1; ++r) {
+    if (t < kCols) {
+      const size_t o = (size_t)r * kP + cl;
+#pragma unroll
+      for (int k = 0; k < kIn; k++) sh[k][t] = f.in[k][o];
+      scl[t] = f.cls[o];
+    }
+    __syncthreads();
+    float s = scl[my];
+#pragma unroll
+    for (int k = 0; k < kIn; k++) s += sh[k][my] + 0.5f * (sh[k][lw] - sh[k][le]);
+    __syncthreads();
+    acc = acc * 0.5f + s;
+    if (own && r >= gfirst) {
+      const size_t o = (size_t)r * kP + cst;
 #pragma unroll
       for (int k = 0; k < kOut; k++) f.out[k][o] = acc + (float)k;
     }
@@ -123,6 +174,47 @@ __global__ __launch_bounds__(256) void aos(const float4* a, const unsigned char*
       __builtin_nontemporal_store(r.w, &b[i].w);
     } else {
       b[i] = r;
+    }
+  }
+}
+
+// One wavefront per 64-column segment (nseg segments, grid-stride), fields in separate
+// arrays of 64 * nseg elements.
+template <bool NT>
+__global__ __launch_bounds__(256) void seg_soa(Fields f, size_t nseg) {
+  const int lane = threadIdx.x & 63;
+  const size_t waves = ((size_t)gridDim.x * blockDim.x) >> 6;
+  for (size_t sg = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; sg < nseg; sg += waves) {
+    const size_t o = sg * 64 + lane;
+    const float h = f.in[0][o], qx = f.in[1][o], qy = f.in[2][o], b = f.in[3][o];
+    const float c = f.cls[o];
+    const float r[3] = {h + b + c, qx * b, qy - c};
+#pragma unroll
+    for (int k = 0; k < kOut; k++) {
+      if (NT) __builtin_nontemporal_store(r[k], &f.out[k][o]);
+      else f.out[k][o] = r[k];
+    }
+  }
+}
+
+// The same with h, qx, qy of segment sg at hqq[192 sg + 64 k + lane] (k = 0, 1, 2), bed and
+// the class at [64 sg + lane], the result at out[192 sg + 64 k + lane].
+template <bool NT>
+__global__ __launch_bounds__(256) void seg_aosoa(const float* hqq, const float* bed, const unsigned char* cls,
+                                                 float* out, size_t nseg) {
+  const int lane = threadIdx.x & 63;
+  const size_t waves = ((size_t)gridDim.x * blockDim.x) >> 6;
+  for (size_t sg = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; sg < nseg; sg += waves) {
+    const float* src = hqq + sg * 192 + lane;
+    const float h = src[0], qx = src[64], qy = src[128];
+    const float b = bed[sg * 64 + lane];
+    const float c = cls[sg * 64 + lane];
+    const float r[3] = {h + b + c, qx * b, qy - c};
+    float* d = out + sg * 192 + lane;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (NT) __builtin_nontemporal_store(r[k], d + 64 * k);
+      else d[64 * k] = r[k];
     }
   }
 }
@@ -224,6 +316,21 @@ int main() {
       std::printf("| march/%d in step | %d-row spans | %.4f | %.0f |\n", wpb, span, ms, bytes / ms / 1e6);
     }
   }
+  for (int span : {32, 160}) {
+    const int nsp = (kH + span - 1) / span, groups = (tiles_x + 7) / 8;
+    const float ms = time_ms([&] {
+      hipLaunchKernelGGL(march_lds, dim3(nsp * groups), dim3(512), 0, 0, f, span, groups);
+    }, reps);
+    std::printf("| march/lds | %d-row spans, 8 waves/group, rows staged in LDS | %.4f | %.0f |\n", span, ms,
+                bytes / ms / 1e6);
+  }
+  for (int span : {160}) {
+    const int nsp = (kH + span - 1) / span, groups = (tiles_x + 7) / 8;
+    const float ms = time_ms([&] {
+      hipLaunchKernelGGL(march<true>, dim3(nsp * groups), dim3(512), 0, 0, f, span, tiles_x, groups);
+    }, reps);
+    std::printf("| march/8 in step | %d-row spans | %.4f | %.0f |\n", span, ms, bytes / ms / 1e6);
+  }
   const size_t n4 = (size_t)kW * kH / 4;
   for (int blocks : {2048, 8192}) {
     const float ms = time_ms([&] { hipLaunchKernelGGL(flat, dim3(blocks), dim3(256), 0, 0, f, n4); }, reps);
@@ -289,6 +396,31 @@ int main() {
                   29.0 * 4 * n4 / m / 1e6);
     }
     CK(hipFree(big));
+  }
+  {
+    // 64-column segments over the pitch: the SoA fields above hold kP * kH elements each
+    static_assert(kP % 64 == 0, "segments cover the pitch");
+    const size_t nseg = (size_t)kP / 64 * kH;
+    float *hqq = nullptr, *out = nullptr;
+    CK(hipMalloc(&hqq, 3 * n * sizeof(float)));
+    CK(hipMalloc(&out, 3 * n * sizeof(float)));
+    CK(hipMemset(hqq, 0, 3 * n * sizeof(float)));
+    for (int blocks : {2048, 8192}) {
+      m = time_ms([&] { hipLaunchKernelGGL(seg_soa<false>, dim3(blocks), dim3(256), 0, 0, f, nseg); }, reps);
+      std::printf("| seg/soa | %d blocks of 256, 8 streams | %.4f | %.0f |\n", blocks, m, bytes / m / 1e6);
+      m = time_ms([&] { hipLaunchKernelGGL(seg_soa<true>, dim3(blocks), dim3(256), 0, 0, f, nseg); }, reps);
+      std::printf("| seg/soa/nt | %d blocks of 256, 8 streams | %.4f | %.0f |\n", blocks, m, bytes / m / 1e6);
+      m = time_ms([&] {
+        hipLaunchKernelGGL(seg_aosoa<false>, dim3(blocks), dim3(256), 0, 0, hqq, f.in[3], f.cls, out, nseg);
+      }, reps);
+      std::printf("| seg/aosoa | %d blocks of 256, 4 streams | %.4f | %.0f |\n", blocks, m, bytes / m / 1e6);
+      m = time_ms([&] {
+        hipLaunchKernelGGL(seg_aosoa<true>, dim3(blocks), dim3(256), 0, 0, hqq, f.in[3], f.cls, out, nseg);
+      }, reps);
+      std::printf("| seg/aosoa/nt | %d blocks of 256, 4 streams | %.4f | %.0f |\n", blocks, m, bytes / m / 1e6);
+    }
+    CK(hipFree(hqq));
+    CK(hipFree(out));
   }
   CK(hipDeviceSynchronize());
   return 0;

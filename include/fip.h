@@ -172,7 +172,8 @@ int fip_tiles(fip_sim* sim, int* tiles_x, int* tiles_y, int* tile_cols, int* til
 int fip_get_masks(fip_sim* sim, uint64_t* dst, int n);
 int fip_get_cfl(fip_sim* sim, float* cfl_min);
 /* Launch shape of the stencil: wavefronts per work-group (1..16; 0 = the default: 16 for
- * the dense fast variant, 4 otherwise) and, in masked mode, persistent work-groups per CU
+ * the dense fast variant, 1 for the dense exact variant, 4 in masked mode) and, in masked
+ * mode, persistent work-groups per CU
  * (1..64), or 0 for a grid of one wavefront per active-tile entry (< 0 = default: 0).
  * In dense mode the wavefronts of a work-group take adjacent tile columns (the fast
  * variant marches them in step). Results do not depend on it. */
