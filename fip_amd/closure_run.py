@@ -244,9 +244,31 @@ def time_ensemble_step(steps=300, n=8, W=1024, warm=200, sat=1e30, seed=11, form
     return tuple(out)
 
 
+def time_training(steps=200, scenarios=2, W=1024, duration=600.0, batch=32, patch=128, lr=1e-3, log=print):
+    """Wall time of `steps` training steps (batch x patch^2 coarse pixels each: forward,
+    loss, backward, Adam) on samples of `scenarios` fine runs of W^2, after 10 warm-up
+    steps. Returns a dict."""
+    from .sim import Closure
+    n = scenarios * int(duration // 60.0)
+    cl = Closure(W // F, W // F, max_samples=n, seed=11)
+    generate_samples(cl, range(scenarios), duration, 60.0, W=W, log=log)
+    cl.train(10, batch=batch, patch=patch, lr=lr)
+    t0 = time.perf_counter()
+    losses = cl.train(steps, batch=batch, patch=patch, lr=lr, seed=9)
+    wall = time.perf_counter() - t0
+    cl.close()
+    out = {"train_steps": steps, "batch": batch, "patch": patch, "samples": n, "wall_s": round(wall, 3),
+           "ms_per_step": round(wall / steps * 1e3, 3), "pixels_per_step": batch * patch * patch,
+           "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
+    log(str(out))
+    return out
+
+
 if __name__ == "__main__":
     import argparse
-    ap = argparse.ArgumentParser(description="time the coarse ensemble step with and without the closure")
+    ap = argparse.ArgumentParser(description="time the coarse ensemble step with and without the closure, "
+                                             "or (--train-steps) the training step")
+    ap.add_argument("--train-steps", type=int, default=0, help="time this many training steps instead")
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--scenarios", type=int, default=8)
     ap.add_argument("--fine", type=int, default=1024, help="fine grid size (the coarse one is a quarter)")
@@ -254,4 +276,7 @@ if __name__ == "__main__":
     ap.add_argument("--form", type=int, default=None, help="fused kernel form (fip_closure_set_kernel)")
     ap.add_argument("--sat", type=float, default=1e30, help="saturation limit of the network outputs")
     a = ap.parse_args()
-    time_ensemble_step(a.steps, a.scenarios, a.fine, a.warm, sat=a.sat, form=a.form)
+    if a.train_steps:
+        time_training(a.train_steps)
+    else:
+        time_ensemble_step(a.steps, a.scenarios, a.fine, a.warm, sat=a.sat, form=a.form)

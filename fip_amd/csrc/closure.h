@@ -76,6 +76,7 @@ hipError_t cl_reduce_launch(const float* part, int nparts, int n, float* out, hi
 hipError_t cl_loss_launch(const float* o, const float* qp, const float* qt, const uint8_t* mask, long npix,
                           float bound, float inv_count, u16* dz, double* part, hipStream_t st);
 hipError_t cl_pack_launch(const float* w, int nlayers, u16* wf, u16* wb, hipStream_t st);
+hipError_t cl_loss_sum_launch(const double* part, int nb, float* out, hipStream_t st);
 hipError_t cl_adam_launch(float* w, float* m, float* v, const float* g, int nlayers, float lr, float b1,
                           float b2, float eps, float c1, float c2, hipStream_t st);
 // Correction of the current face discharges by o; saturated faces counted per work-group

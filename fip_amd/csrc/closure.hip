@@ -314,13 +314,32 @@ extern "C" __global__ __launch_bounds__(256) void cl_wgrad(const u16* __restrict
   if (threadIdx.x < CL_C) pb[CL_TAPS * CL_C * CL_C + threadIdx.x] = dbias;
 }
 
-// out[i] = sum_b part[b][i] (fixed order: deterministic)
-extern "C" __global__ void cl_reduce(const float* part, int nparts, int n, float* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// out[i] = sum_b part[b][i] in a fixed order (deterministic): a work-group of 16 wavefronts
+// takes 64 outputs; wavefront w adds parts w, w + 16, w + 32, ... in turn, with its loads
+// issued 8 at a time, and the 16 wavefront sums are added in wavefront order.
+#define RED_WAVES 16
+extern "C" __global__ __launch_bounds__(64 * RED_WAVES) void cl_reduce(const float* __restrict__ part, int nparts,
+                                                                      int n, float* __restrict__ out) {
+  __shared__ float red[RED_WAVES][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
   float s = 0.f;
-  for (int b = 0; b < nparts; b++) s += part[(long)b * n + i];
-  out[i] = s;
+  if (i < n) {
+    int b = wave;
+    for (; b + 7 * RED_WAVES < nparts; b += 8 * RED_WAVES) {
+      float v[8];
+      for (int k = 0; k < 8; k++) v[k] = part[(long)(b + k * RED_WAVES) * n + i];
+      for (int k = 0; k < 8; k++) s += v[k];
+    }
+    for (; b < nparts; b += RED_WAVES) s += part[(long)b * n + i];
+  }
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && i < n) {
+    float t = red[0][lane];
+    for (int w = 1; w < RED_WAVES; w++) t += red[w][lane];
+    out[i] = t;
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -363,6 +382,29 @@ extern "C" __global__ void cl_loss(const float* o, const float* qp, const float*
   }
   if (threadIdx.x == 0)
     for (int k = 0; k < 3; k++) part[(long)blockIdx.x * 3 + k] = sh[k][0];
+}
+
+// The batch loss of a training step from cl_loss's partial sums (nb blocks of (m r^2, m,
+// m (qp - qt)^2)): out = sum m r^2 / sum m, or 0 without a trainable face. One work-group
+// of 256 threads in a fixed order, so the value does not depend on timing.
+extern "C" __global__ void cl_loss_sum(const double* part, int nb, float* out) {
+  __shared__ double sh[2][256];
+  double r = 0.0, m = 0.0;
+  for (int b = (int)threadIdx.x; b < nb; b += (int)blockDim.x) {
+    r += part[(long)b * 3];
+    m += part[(long)b * 3 + 1];
+  }
+  sh[0][threadIdx.x] = r;
+  sh[1][threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[1][0] > 0.0 ? (float)(sh[0][0] / sh[1][0]) : 0.f;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1391,12 +1433,13 @@ CL_WRAP(cl_feat_from_float, (const float* f, long npix, u16* out, hipStream_t st
 CL_WRAP(cl_wgrad, (const u16* X, const u16* dZ, float* part, int N, int H, int W, int nblocks, hipStream_t st),
         P(X, dZ, part, N, H, W), dim3(nblocks), dim3(256))
 CL_WRAP(cl_reduce, (const float* part, int nparts, int n, float* out, hipStream_t st), P(part, nparts, n, out),
-        dim3((n + 255) / 256), dim3(256))
+        dim3((n + 63) / 64), dim3(64 * RED_WAVES))
 CL_WRAP(cl_loss, (const float* o, const float* qp, const float* qt, const uint8_t* mask, long npix, float bound,
                   float inv_count, u16* dz, double* part, hipStream_t st),
         P(o, qp, qt, mask, npix, bound, inv_count, dz, part), dim3((unsigned)((npix + 255) / 256)), dim3(256))
 CL_WRAP(cl_pack, (const float* w, int nlayers, u16* wf, u16* wb, hipStream_t st), P(w, nlayers, wf, wb),
         dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))
+CL_WRAP(cl_loss_sum, (const double* part, int nb, float* out, hipStream_t st), P(part, nb, out), dim3(1), dim3(256))
 CL_WRAP(cl_adam, (float* w, float* m, float* v, const float* g, int nlayers, float lr, float b1, float b2,
                   float eps, float c1, float c2, hipStream_t st),
         P(w, m, v, g, nlayers, lr, b1, b2, eps, c1, c2), dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))

@@ -27,7 +27,7 @@ using fip_internal::fail;
 namespace {
 
 constexpr float kBound = 0.1f;   // |c| <= 0.1 |q|
-constexpr int kWgradBlocks = 256;
+constexpr int kWgradBlocks = 768;  // 3 work-groups of 42.5 KB LDS per CU
 
 // splitmix64 stream with uniform and normal draws (Box-Muller)
 struct Rng {
@@ -454,30 +454,53 @@ int fip_closure_train(fip_closure* cl, int steps, int batch, int patch, float lr
   if (n < 1 || first < 0 || first + n > cl->ns) return fail(FIP_E_INVALID, "sample range outside the store");
   if (patch < 8 || patch > std::min(cl->Wc, cl->Hc) || batch < 1)
     return fail(FIP_E_INVALID, "patch must fit the coarse grid");
+  if (steps < 0) return fail(FIP_E_INVALID, "steps must not be negative");
+  if (steps == 0) return FIP_OK;
   CHIP(hipSetDevice(cl->dev));
   int rc = cl->train.ensure(batch, patch, patch, true);
   if (rc) return rc;
+  // every step's batch origins are drawn up front and uploaded once, and every step's loss
+  // stays on the device until the end: the steps run back to back with no host round trip
   Rng rng{seed};
-  std::vector<int> org((size_t)batch * 3);
-  const size_t spx = (size_t)cl->Wc * cl->Hc;
-  for (int it = 0; it < steps; it++) {
+  std::vector<int> org((size_t)steps * batch * 3);
+  for (int it = 0; it < steps; it++)
     for (int b = 0; b < batch; b++) {
-      org[(size_t)b * 3] = first + (int)(rng.uniform() * n) % n;
-      org[(size_t)b * 3 + 1] = (int)(rng.uniform() * (cl->Wc - patch + 1)) % (cl->Wc - patch + 1);
-      org[(size_t)b * 3 + 2] = (int)(rng.uniform() * (cl->Hc - patch + 1)) % (cl->Hc - patch + 1);
+      int* o = &org[((size_t)it * batch + b) * 3];
+      o[0] = first + (int)(rng.uniform() * n) % n;
+      o[1] = (int)(rng.uniform() * (cl->Wc - patch + 1)) % (cl->Wc - patch + 1);
+      o[2] = (int)(rng.uniform() * (cl->Hc - patch + 1)) % (cl->Hc - patch + 1);
     }
-    Work& ws = cl->train;
-    CHIP(hipMemcpyAsync(ws.origin, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, cl->stream));
-    CHIP(cl_gather_launch(cl->s_feat, (long)spx * 8, cl->Wc, ws.origin, patch, patch, ws.x0, batch, cl->stream));
-    CHIP(cl_gather_targets_launch(cl->s_qp, cl->s_qt, cl->s_mask, (long)spx, cl->Wc, ws.origin, patch, patch,
-                                  ws.qp, ws.qt, ws.mask, batch, cl->stream));
-    if ((rc = forward(cl, ws, nullptr, 0, cl->stream))) return rc;
-    if ((rc = backward(cl, ws, kBound, cl->stream))) return rc;
-    if ((rc = adam_update(cl, lr))) return rc;
-    double r, m, p;
-    if ((rc = read_loss(ws, cl->stream, &r, &m, &p))) return rc;
-    if (losses) losses[it] = m > 0 ? (float)(r / m) : 0.f;
+  int* d_org = nullptr;
+  float* d_loss = nullptr;
+  if ((rc = dalloc(&d_org, org.size()))) return rc;
+  if ((rc = dalloc(&d_loss, (size_t)steps))) {
+    hipFree(d_org);
+    return rc;
   }
+  const size_t spx = (size_t)cl->Wc * cl->Hc;
+  Work& ws = cl->train;
+  const int nb = (int)((ws.pixels() + 255) / 256);
+  hipError_t e = hipMemcpyAsync(d_org, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, cl->stream);
+  for (int it = 0; it < steps && e == hipSuccess && !rc; it++) {
+    const int* o = d_org + (size_t)it * batch * 3;
+    e = cl_gather_launch(cl->s_feat, (long)spx * 8, cl->Wc, o, patch, patch, ws.x0, batch, cl->stream);
+    if (e == hipSuccess)
+      e = cl_gather_targets_launch(cl->s_qp, cl->s_qt, cl->s_mask, (long)spx, cl->Wc, o, patch, patch, ws.qp,
+                                   ws.qt, ws.mask, batch, cl->stream);
+    if (e != hipSuccess) break;
+    if ((rc = forward(cl, ws, nullptr, 0, cl->stream))) break;
+    if ((rc = backward(cl, ws, kBound, cl->stream))) break;
+    e = cl_loss_sum_launch(ws.lpart, nb, d_loss + it, cl->stream);
+    if (e == hipSuccess) rc = adam_update(cl, lr);
+  }
+  if (e == hipSuccess && !rc && losses)
+    e = hipMemcpyAsync(losses, d_loss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost, cl->stream);
+  const hipError_t es = hipStreamSynchronize(cl->stream);
+  hipFree(d_org);
+  hipFree(d_loss);
+  if (rc) return rc;
+  if (e != hipSuccess || es != hipSuccess)
+    return fail(FIP_E_HIP, std::string("train: ") + hipGetErrorString(e != hipSuccess ? e : es));
   return FIP_OK;
 }
 

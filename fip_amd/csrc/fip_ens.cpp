@@ -403,6 +403,7 @@ int fip_ens_create(const fip_desc* descs, int n, fip_ens** out) {
     e->sims.push_back(sim);
   }
   if (hipSetDevice(e->dev) != hipSuccess) return cleanup(fail(FIP_E_HIP, "hipSetDevice failed"));
+  e->graph_failed = !fip_internal::graphs_enabled();
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipStreamCreate failed"));
   if (hipMalloc(&e->d_scn, n * sizeof(fip_ens_scn)) != hipSuccess ||

@@ -14,6 +14,10 @@
 namespace fip_internal {
 
 int fail(int code, const std::string& msg);
+// false when the environment sets FIP_GRAPHS=0: the device-resident loops then launch every
+// kernel from the host instead of replaying captured hipGraphs (results are the same; for
+// profilers that cannot trace graph launches).
+bool graphs_enabled();
 // The domain, the solver flag and the stream of a simulation.
 const fip_grid& grid(const fip_sim* s);
 int staggered(const fip_sim* s);

@@ -5,6 +5,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -646,6 +647,7 @@ int fip_create(const fip_desc* d, fip_sim** out) {
   if (hipGetDeviceProperties(&prop, s->dev) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipGetDeviceProperties failed"));
   s->arch = prop.gcnArchName;
+  s->graph_failed = !fip_internal::graphs_enabled();
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipStreamCreate failed"));
 
@@ -1280,6 +1282,11 @@ int fail(int code, const std::string& msg) { return ::fail(code, msg); }
 const fip_grid& grid(const fip_sim* s) { return s->g; }
 int staggered(const fip_sim* s) { return s->d.staggered; }
 int device(const fip_sim* s) { return s->dev; }
+
+bool graphs_enabled() {
+  const char* v = std::getenv("FIP_GRAPHS");
+  return !(v && v[0] == '0' && v[1] == 0);
+}
 
 void set_post_hook(fip_sim* s, StepHook hook) {
   s->post_hook = std::move(hook);
