@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
+#include <type_traits>
 
 #include "closure.h"
 #include "fip_ctl.h"
@@ -1129,31 +1130,40 @@ struct RmCtx {
   int x0, y0, W, H;  // strip origin, domain size
 };
 
-// matrix part of one layer row: NC K chunks, MT output blocks, for region column c
-// (clamped to the region) of row y; L1: features ring input
-template <int NC, int MT, bool L1>
-__device__ __forceinline__ void rm_mfma(const u16* in, int y, int c, const bf16x8 (&a)[NC][MT], f32x4 (&acc)[MT],
-                                        int grp) {
+// Rows live in ring slot (row - ya) & 3, ya the first row of the work-group's span: at
+// march step k every slot is a compile-time function of k & 3 (the march is unrolled by
+// 4), so the LDS accesses below take immediate offsets from a per-lane base.
+
+// matrix part of one layer row (L >= 1) whose row sits in slot S0: NC = 9 K chunks (the
+// taps), MT output blocks; inl = in + ((c + 1) * FPS + 8 grp) for the lane's column c
+template <int NC, int MT, int S0>
+__device__ __forceinline__ void rm_mfma(const u16* inl, const bf16x8 (&a)[NC][MT], f32x4 (&acc)[MT]) {
 #pragma unroll
   for (int cc = 0; cc < NC; cc++) {
-    bf16x8 b;
-    if constexpr (L1) {
-      const int tap = 4 * cc + grp;
-      const int dy = tap < CL_TAPS ? tap / 3 - 1 : 0, dx = tap < CL_TAPS ? tap % 3 - 1 : 0;
-      const bf16x8 v = ld8(in + (((y + dy) & 3) * RFW + c + 1 + dx) * 8);
-      b = tap < CL_TAPS ? v : zero8();
-    } else {
-      const int dy = cc / 3 - 1, dx = cc % 3 - 1;
-      b = ld8(in + (((y + dy) & 3) * RPX + c + 1 + dx) * FPS + 8 * grp);
-    }
+    const int dy = cc / 3 - 1, dx = cc % 3 - 1;
+    const bf16x8 b = ld8(inl + (((S0 + 4 + dy) & 3) * RPX + dx) * FPS);
 #pragma unroll
     for (int m = 0; m < MT; m++) acc[m] = mfma(a[cc][m], b, acc[m]);
   }
 }
 
+// layer 0's matrix part: K chunk cc holds tap 4 cc + grp (8 feature slots each; taps
+// past 8 are zero); off[cc][s]: the lane's feature-ring offset of its tap for a row in slot s
+template <int S0>
+__device__ __forceinline__ void rm_mfma_l0(const u16* in, const int (&off)[3][4], bool tap8,
+                                           const bf16x8 (&a)[3][2], f32x4 (&acc)[2]) {
+#pragma unroll
+  for (int cc = 0; cc < 3; cc++) {
+    const bf16x8 v = ld8(in + off[cc][S0]);
+    const bf16x8 b = (cc < 2 || tap8) ? v : zero8();
+#pragma unroll
+    for (int m = 0; m < 2; m++) acc[m] = mfma(a[cc][m], b, acc[m]);
+  }
+}
+
 // GELU epilogue of one layer row into its ring: region column c of a region of width RW
 // whose column 0 sits at x = x0 + c0off; lanes past the region store to the spare slot
-template <int RW>
+template <int RW, int S0>
 __device__ __forceinline__ void rm_store(u16* out, int y, int c, int x, int W, int H, const f32x4 (&acc)[2],
                                          int grp) {
   const bool valid = c < RW;
@@ -1165,7 +1175,7 @@ __device__ __forceinline__ void rm_store(u16* out, int y, int c, int x, int W, i
     const int co0 = m * 16 + 4 * grp;
     const uint32_t lo = pack_bf2(gelu(acc[m][0]), gelu(acc[m][1])) & keep;
     const uint32_t hi = pack_bf2(gelu(acc[m][2]), gelu(acc[m][3])) & keep;
-    *(uint2*)(out + ((y & 3) * RPX + px) * FPS + co0) = make_uint2(lo, hi);
+    *(uint2*)(out + (S0 * RPX + px) * FPS + co0) = make_uint2(lo, hi);
   }
 }
 
@@ -1218,9 +1228,23 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
   };
   if (L == 3) load_row(ya - 4);
   const int K = nrows + 12;
-  for (int k = 0; k < K; k++) {
+  const u16* inl = in + (cc + 1) * FPS + 8 * grp;  // layers 1..3: the lane's B-fragment base
+  int l0off[3][4];                                   // layer 0: see rm_mfma_l0
+  if constexpr (L == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int tap = 4 * q + grp;
+      const int dy = tap < CL_TAPS ? tap / 3 - 1 : 0, dx = tap < CL_TAPS ? tap % 3 - 1 : 0;
+#pragma unroll
+      for (int s0 = 0; s0 < 4; s0++) l0off[q][s0] = (((s0 + 4 + dy) & 3) * RFW + cc + 1 + dx) * 8;
+    }
+  }
+  const bool tap8 = grp == 0;  // chunk 2 holds tap 8 in lane group 0 only
+  const auto step = [&](auto phase, int k) {
+    constexpr int PH = decltype(phase)::value;       // k & 3
+    constexpr int S0 = ((PH - M) % 4 + 4) % 4;       // slot of this layer's row y = ya - M + k
     if constexpr (L == 3) {
-      const int yF = ya - 4 + k;
+      const int yF = ya - 4 + k;                     // slot (yF - ya) & 3 = PH
       if (k < nrows + 8 && loader) {  // features of row yF (the formulas of feat_pixel)
         uint4 pk = make_uint4(0, 0, 0, 0);
         if (fx >= 0 && fx < W && yF >= 0 && yF < H) {
@@ -1232,7 +1256,7 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
           pk.y = (uint32_t)f2bf(rqy) | ((uint32_t)f2bf(f3v) << 16);
           pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
         }
-        *(uint4*)(out + ((yF & 3) * RFW + fc) * 8) = pk;  // layer 3's out: the feature ring
+        *(uint4*)(out + (PH * RFW + fc) * 8) = pk;  // layer 3's out: the feature ring
         if (fc >= 4 && fc < RS + 4) {
           float* q = raw + ((yF & (RRAW - 1)) * RS + fc - 4) * 3;
           q[0] = rqx;
@@ -1247,12 +1271,12 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
       f32x4 acc[MT];
 #pragma unroll
       for (int m = 0; m < MT; m++) acc[m] = bias[m];
-      if constexpr (L == 3)
-        rm_mfma<NC, MT, false>(in, y, cc, a, acc, grp);
+      if constexpr (L == 0)
+        rm_mfma_l0<S0>(in, l0off, tap8, a, acc);
       else
-        rm_mfma<NC, MT, L == 0>(in, y, cc, a, acc, grp);
+        rm_mfma<NC, MT, S0>(inl, a, acc);
       if constexpr (L < 3) {
-        rm_store<RW>(out, y, c, x0 + c - HALO, W, H, acc, grp);
+        rm_store<RW, S0>(out, y, c, x0 + c - HALO, W, H, acc, grp);
       } else {  // outputs 0 and 1 (lane group 0) of row y, and its mass terms
         const int x = x0 + c;
         if (grp == 0 && c < RS && x < W) {
@@ -1271,6 +1295,12 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
       }
     }
     __syncthreads();
+  };
+  for (int k0 = 0; k0 < K; k0 += 4) {  // K is the same in every wavefront of the work-group
+    step(std::integral_constant<int, 0>{}, k0);
+    if (k0 + 1 < K) step(std::integral_constant<int, 1>{}, k0 + 1);
+    if (k0 + 2 < K) step(std::integral_constant<int, 2>{}, k0 + 2);
+    if (k0 + 3 < K) step(std::integral_constant<int, 3>{}, k0 + 3);
   }
 }
 
