@@ -1110,59 +1110,74 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
 // ---------------------------------------------------------------------------------------
 // Row-marching form of the fused inference step (form 2). A work-group takes a strip of
 // RS = 58 output columns and a span of rows [ya, yb) of one domain and marches along its
-// rows. Each layer keeps its last four rows in an LDS ring, so every row of every layer
-// is computed once: no vertical halo is recomputed, and the horizontal halo is 3 columns
-// per side at layer 0 (the tile forms recompute a 3-cell halo on all four sides). March
-// step k loads the features of row ya - 4 + k and layer L computes row ya - (6 + 2 L) + k,
-// so every input row was written in an earlier step; one barrier per step. 16 wavefronts: wavefront 4 L + j runs
-// layer L on pixel tile j (16 columns), and the four wavefronts on a SIMD are one of each
-// layer. Each keeps the A fragments of its layer in registers for the whole march. Every
-// output sums the same fragments in the same order as cl_fused: the same bits. The mass
-// terms come from the loaded discharges and depths (a 16-row ring); cl_correct(_ens)
-// applies the outputs.
+// rows, two rows per march step. Each layer keeps its last eight rows in an LDS ring, so
+// every row of every layer is computed once: no vertical halo is recomputed, and the
+// horizontal halo is 3 columns per side at layer 0 (the tile forms recompute a 3-cell
+// halo on all four sides). March step k loads the features of rows ya - 4 + 2k and
+// ya - 3 + 2k, and layer L computes rows ya - (8 + 4 L) + 2k and the row after it, so
+// every input row was written in an earlier step; one barrier per step. 16 wavefronts:
+// wavefront 4 L + j runs layer L on pixel tile j (16 columns), and the four wavefronts on
+// a SIMD are one of each layer. Each keeps the A fragments of its layer in registers for
+// the whole march, and each B fragment it reads feeds both of its rows. Every output sums
+// the same fragments in the same order as cl_fused: the same bits. The lanes that load
+// the features also add the mass terms of their rows; cl_correct(_ens) applies the
+// outputs.
 #define RS 58                  // output columns per strip
 #define RPX 65                 // pixel slots per ring row: 4 tiles of 16 + a spare slot
 #define RFW (RS + 8)           // feature columns (4 on each side)
-#define RRAW 16                // rows of the raw ring
+#define RROWS 8                // rows per ring
 
-struct RmCtx {
-  const u16* fring;  // [4][RFW][8]
-  int x0, y0, W, H;  // strip origin, domain size
-};
-
-// Rows live in ring slot (row - ya) & 3, ya the first row of the work-group's span: at
+// Rows live in ring slot (row - ya) & 7, ya the first row of the work-group's span: at
 // march step k every slot is a compile-time function of k & 3 (the march is unrolled by
 // 4), so the LDS accesses below take immediate offsets from a per-lane base.
+__host__ __device__ constexpr int rm_slot(int r) { return ((r % RROWS) + RROWS) % RROWS; }
 
-// matrix part of one layer row (L >= 1) whose row sits in slot S0: NC = 9 K chunks (the
-// taps), MT output blocks; inl = in + ((c + 1) * FPS + 8 grp) for the lane's column c
-template <int NC, int MT, int S0>
-__device__ __forceinline__ void rm_mfma(const u16* inl, const bf16x8 (&a)[NC][MT], f32x4 (&acc)[MT]) {
+// matrix part of rows y (slot S0) and y + 1 of a layer L >= 1: 9 taps, MT output blocks;
+// inl = in + ((c + 1) * FPS + 8 grp) for the lane's column c. Input row y + d (d = -1..2)
+// at column shift dx is one B fragment: tap (d, dx) of row y and tap (d - 1, dx) of row
+// y + 1 use it, and each accumulator still takes its taps in order 0..8.
+template <int MT, int S0>
+__device__ __forceinline__ void rm_mfma2(const u16* inl, const bf16x8 (&a)[CL_TAPS][MT], f32x4 (&acc0)[MT],
+                                         f32x4 (&acc1)[MT]) {
 #pragma unroll
-  for (int cc = 0; cc < NC; cc++) {
-    const int dy = cc / 3 - 1, dx = cc % 3 - 1;
-    const bf16x8 b = ld8(inl + (((S0 + 4 + dy) & 3) * RPX + dx) * FPS);
+  for (int d = -1; d <= 2; d++) {
 #pragma unroll
-    for (int m = 0; m < MT; m++) acc[m] = mfma(a[cc][m], b, acc[m]);
+    for (int dx = -1; dx <= 1; dx++) {
+      const bf16x8 b = ld8(inl + (rm_slot(S0 + d) * RPX + dx) * FPS);
+      if (d <= 1) {
+#pragma unroll
+        for (int m = 0; m < MT; m++) acc0[m] = mfma(a[(d + 1) * 3 + dx + 1][m], b, acc0[m]);
+      }
+      if (d >= 0) {
+#pragma unroll
+        for (int m = 0; m < MT; m++) acc1[m] = mfma(a[d * 3 + dx + 1][m], b, acc1[m]);
+      }
+    }
   }
 }
 
-// layer 0's matrix part: K chunk cc holds tap 4 cc + grp (8 feature slots each; taps
-// past 8 are zero); off[cc][s]: the lane's feature-ring offset of its tap for a row in slot s
+// layer 0's matrix part of rows y (slot S0) and y + 1: K chunk cc holds tap 4 cc + grp
+// (8 feature slots each; taps past 8 are zero); off[cc][s]: the lane's feature-ring offset
+// of its tap for a row in slot s
 template <int S0>
-__device__ __forceinline__ void rm_mfma_l0(const u16* in, const int (&off)[3][4], bool tap8,
-                                           const bf16x8 (&a)[3][2], f32x4 (&acc)[2]) {
+__device__ __forceinline__ void rm_mfma2_l0(const u16* in, const int (&off)[3][RROWS], bool tap8,
+                                            const bf16x8 (&a)[3][2], f32x4 (&acc0)[2], f32x4 (&acc1)[2]) {
 #pragma unroll
   for (int cc = 0; cc < 3; cc++) {
-    const bf16x8 v = ld8(in + off[cc][S0]);
-    const bf16x8 b = (cc < 2 || tap8) ? v : zero8();
+    const bf16x8 v0 = ld8(in + off[cc][S0]);
+    const bf16x8 v1 = ld8(in + off[cc][rm_slot(S0 + 1)]);
+    const bool ok = cc < 2 || tap8;
+    const bf16x8 b0 = ok ? v0 : zero8(), b1 = ok ? v1 : zero8();
 #pragma unroll
-    for (int m = 0; m < 2; m++) acc[m] = mfma(a[cc][m], b, acc[m]);
+    for (int m = 0; m < 2; m++) {
+      acc0[m] = mfma(a[cc][m], b0, acc0[m]);
+      acc1[m] = mfma(a[cc][m], b1, acc1[m]);
+    }
   }
 }
 
-// GELU epilogue of one layer row into its ring: region column c of a region of width RW
-// whose column 0 sits at x = x0 + c0off; lanes past the region store to the spare slot
+// GELU epilogue of one layer row into its ring (slot S0): region column c of a region of
+// width RW; lanes past the region store to the spare slot
 template <int RW, int S0>
 __device__ __forceinline__ void rm_store(u16* out, int y, int c, int x, int W, int H, const f32x4 (&acc)[2],
                                          int grp) {
@@ -1186,17 +1201,22 @@ __device__ __forceinline__ void rm_load_a(const u16* wp, int NC, int MT, int lan
 
 #define RNW 16                 // wavefronts per work-group of cl_fused_r: 4 per layer
 
+// features of one row for one loader lane, and the row's mass terms
+struct RmRow {
+  float b, h, qx, qy, bs, k;
+};
+
 // One wavefront of layer L (0..3) on pixel tile j over the march steps: reads its input
-// ring `in`, writes its output ring `out` (layer 3: the outputs o and the mass terms; it
-// also loads the features of the next row and writes them to `out`, the feature ring).
-// Layer L computes row ya - (6 + 2 L) + k at step k.
+// ring `in`, writes its output ring `out` (layer 3: the outputs o; it also loads the
+// features of the next rows, writes them to `out`, the feature ring, and adds their mass
+// terms). Layer L computes rows ya - (8 + 4 L) + 2k and the next at step k.
 template <int L>
 __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u16* in, u16* out, int j, int lane,
-                                         int x0, int ya, int nrows, int W, int H, float* od, float* raw,
-                                         double& v, double& fl, bool mass, const ClDom& D, int cur) {
+                                         int x0, int ya, int nrows, int W, int H, float* od, double& v, double& fl,
+                                         bool mass, const ClDom& D, int cur) {
   constexpr int NC = L == 0 ? 3 : CL_TAPS, MT = L == 3 ? 1 : 2;
   constexpr int RW = RS + 6 - 2 * L;       // region width
-  constexpr int M = 6 + 2 * L;             // row of step k: ya - M + k
+  constexpr int M = 8 + 4 * L;             // first row of step k: ya - M + 2k
   constexpr int HALO = 3 - L;              // rows the next layer needs: [ya - HALO, yb + HALO)
   const int col = lane & 15, grp = lane >> 4;
   const int c = 16 * j + col;
@@ -1213,83 +1233,99 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
   const int fc = (int)threadIdx.x - 3 * 4 * 64;
   const bool loader = L == 3 && fc >= 0 && fc < RFW;
   const int fx = x0 + fc - 4;
-  float rb = 0.f, rh = 0.f, rqx = 0.f, rqy = 0.f, rbs = 0.f, rk = 0.f;
-  const auto load_row = [&](int yy) {
-    rb = rh = rqx = rqy = rbs = rk = 0.f;
-    if (loader && fx >= 0 && fx < W && yy >= 0 && yy < H) {
+  const bool fxin = fx >= 0 && fx < W;
+  const bool fmass = mass && loader && fc >= 4 && fc < RS + 4 && fx < W;  // a column of the strip
+  RmRow nr[2];
+  const auto load_row = [&](RmRow& r, int yy) {
+    r = RmRow{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (loader && fxin && yy >= 0 && yy < H) {
       const long o = (long)yy * D.g.P + fx;
-      rb = D.g.bed[o];
-      rh = D.g.h[cur][o];
-      rqx = D.g.qx[cur][o];
-      rqy = D.g.qy[cur][o];
-      rbs = D.bstd[(long)yy * W + fx];
-      rk = D.g.kst ? D.g.kst[o] : D.g.kst_uniform;
+      r.b = D.g.bed[o];
+      r.h = D.g.h[cur][o];
+      r.qx = D.g.qx[cur][o];
+      r.qy = D.g.qy[cur][o];
+      r.bs = D.bstd[(long)yy * W + fx];
+      r.k = D.g.kst ? D.g.kst[o] : D.g.kst_uniform;
     }
   };
-  if (L == 3) load_row(ya - 4);
-  const int K = nrows + 12;
+  const int nfeat = nrows + 8;             // feature rows [ya - 4, yb + 4)
+  if (L == 3) {
+    load_row(nr[0], ya - 4);
+    load_row(nr[1], ya - 3);
+  }
+  const int K = (nrows + 21) / 2;          // the last pair of layer 3 holds row yb - 1
   const u16* inl = in + (cc + 1) * FPS + 8 * grp;  // layers 1..3: the lane's B-fragment base
-  int l0off[3][4];                                   // layer 0: see rm_mfma_l0
+  int l0off[3][RROWS];                               // layer 0: see rm_mfma2_l0
   if constexpr (L == 0) {
 #pragma unroll
     for (int q = 0; q < 3; q++) {
       const int tap = 4 * q + grp;
       const int dy = tap < CL_TAPS ? tap / 3 - 1 : 0, dx = tap < CL_TAPS ? tap % 3 - 1 : 0;
 #pragma unroll
-      for (int s0 = 0; s0 < 4; s0++) l0off[q][s0] = (((s0 + 4 + dy) & 3) * RFW + cc + 1 + dx) * 8;
+      for (int s0 = 0; s0 < RROWS; s0++) l0off[q][s0] = (((s0 + RROWS + dy) & (RROWS - 1)) * RFW + cc + 1 + dx) * 8;
     }
   }
   const bool tap8 = grp == 0;  // chunk 2 holds tap 8 in lane group 0 only
   const auto step = [&](auto phase, int k) {
     constexpr int PH = decltype(phase)::value;       // k & 3
-    constexpr int S0 = ((PH - M) % 4 + 4) % 4;       // slot of this layer's row y = ya - M + k
+    constexpr int S0 = rm_slot(2 * PH - M);          // slot of this layer's row y = ya - M + 2k
     if constexpr (L == 3) {
-      const int yF = ya - 4 + k;                     // slot (yF - ya) & 3 = PH
-      if (k < nrows + 8 && loader) {  // features of row yF (the formulas of feat_pixel)
-        uint4 pk = make_uint4(0, 0, 0, 0);
-        if (fx >= 0 && fx < W && yF >= 0 && yF < H) {
-          const float f0v = (rh + rb - D.zref) * 0.25f;
-          const float f3v = (rb - D.zref) * 0.25f;
-          const float f4v = rbs * 2.0f;
-          const float f5v = rk * (1.0f / 30.0f);
-          pk.x = (uint32_t)f2bf(f0v) | ((uint32_t)f2bf(rqx) << 16);
-          pk.y = (uint32_t)f2bf(rqy) | ((uint32_t)f2bf(f3v) << 16);
-          pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
+      const int yF = ya - 4 + 2 * k;                 // slot (yF - ya) & 7 = rm_slot(2 PH - 4)
+      if (loader) {
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+          const int yr = yF + r;
+          if (2 * k + r < nfeat) {  // features of row yr (the formulas of feat_pixel)
+            const RmRow& w = nr[r];
+            uint4 pk = make_uint4(0, 0, 0, 0);
+            if (fxin && yr >= 0 && yr < H) {
+              const float f0v = (w.h + w.b - D.zref) * 0.25f;
+              const float f3v = (w.b - D.zref) * 0.25f;
+              const float f4v = w.bs * 2.0f;
+              const float f5v = w.k * (1.0f / 30.0f);
+              pk.x = (uint32_t)f2bf(f0v) | ((uint32_t)f2bf(w.qx) << 16);
+              pk.y = (uint32_t)f2bf(w.qy) | ((uint32_t)f2bf(f3v) << 16);
+              pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
+            }
+            *(uint4*)(out + (rm_slot(2 * PH - 4 + r) * RFW + fc) * 8) = pk;  // the feature ring
+            if (fmass && yr >= ya && yr < ya + nrows) {  // mass terms of an owned cell
+              const bool rows = yr >= 2 && yr <= H - 3, cols = fx >= 2 && fx <= W - 3;
+              if (rows && cols) v += (double)w.h;
+              if (rows && fx == W - 3) fl += (double)w.qx;
+              if (rows && fx == 1) fl -= (double)w.qx;
+              if (cols && yr == H - 3) fl += (double)w.qy;
+              if (cols && yr == 1) fl -= (double)w.qy;
+            }
+          }
         }
-        *(uint4*)(out + (PH * RFW + fc) * 8) = pk;  // layer 3's out: the feature ring
-        if (fc >= 4 && fc < RS + 4) {
-          float* q = raw + ((yF & (RRAW - 1)) * RS + fc - 4) * 3;
-          q[0] = rqx;
-          q[1] = rqy;
-          q[2] = rh;
+        if (2 * k + 2 < nfeat) {  // the next pair lands during this step
+          load_row(nr[0], yF + 2);
+          load_row(nr[1], yF + 3);
         }
       }
-      if (k + 1 < nrows + 8) load_row(yF + 1);  // lands during this step
     }
-    const int y = ya - M + k;
-    if (k >= M - HALO && k < nrows + M + HALO) {
-      f32x4 acc[MT];
+    const int y = ya - M + 2 * k;
+    if (2 * k + 1 - M >= -HALO && 2 * k - M < nrows + HALO) {
+      f32x4 acc0[MT], acc1[MT];
 #pragma unroll
-      for (int m = 0; m < MT; m++) acc[m] = bias[m];
+      for (int m = 0; m < MT; m++) acc0[m] = acc1[m] = bias[m];
       if constexpr (L == 0)
-        rm_mfma_l0<S0>(in, l0off, tap8, a, acc);
+        rm_mfma2_l0<S0>(in, l0off, tap8, a, acc0, acc1);
       else
-        rm_mfma<NC, MT, S0>(inl, a, acc);
+        rm_mfma2<MT, S0>(inl, a, acc0, acc1);
       if constexpr (L < 3) {
-        rm_store<RW, S0>(out, y, c, x0 + c - HALO, W, H, acc, grp);
-      } else {  // outputs 0 and 1 (lane group 0) of row y, and its mass terms
+        rm_store<RW, S0>(out, y, c, x0 + c - HALO, W, H, acc0, grp);
+        rm_store<RW, rm_slot(S0 + 1)>(out, y + 1, c, x0 + c - HALO, W, H, acc1, grp);
+      } else {  // outputs 0 and 1 (lane group 0) of rows y and y + 1 that the span owns
         const int x = x0 + c;
         if (grp == 0 && c < RS && x < W) {
-          od[((long)y * W + x) * 2] = acc[0][0];
-          od[((long)y * W + x) * 2 + 1] = acc[0][1];
-          if (mass) {
-            const float* q = raw + ((y & (RRAW - 1)) * RS + c) * 3;
-            const bool rows = y >= 2 && y <= H - 3, cols = x >= 2 && x <= W - 3;
-            if (rows && cols) v += (double)q[2];
-            if (rows && x == W - 3) fl += (double)q[0];
-            if (rows && x == 1) fl -= (double)q[0];
-            if (cols && y == H - 3) fl += (double)q[1];
-            if (cols && y == 1) fl -= (double)q[1];
+          if (y >= ya && y < ya + nrows) {
+            od[((long)y * W + x) * 2] = acc0[0][0];
+            od[((long)y * W + x) * 2 + 1] = acc0[0][1];
+          }
+          if (y + 1 >= ya && y + 1 < ya + nrows) {
+            od[((long)(y + 1) * W + x) * 2] = acc1[0][0];
+            od[((long)(y + 1) * W + x) * 2 + 1] = acc1[0][1];
           }
         }
       }
@@ -1308,11 +1344,10 @@ extern "C" __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __
                                                              const u16* wf, const u16* wl1, const float* w,
                                                              double* acc_out, float* o_out, int nstrip, int nspan,
                                                              int span) {
-  __shared__ __attribute__((aligned(16))) u16 fring[4 * RFW * 8];
-  __shared__ __attribute__((aligned(16))) u16 ring1[4 * RPX * FPS];
-  __shared__ __attribute__((aligned(16))) u16 ring2[4 * RPX * FPS];
-  __shared__ __attribute__((aligned(16))) u16 ring3[4 * RPX * FPS];
-  __shared__ float raw[RRAW * RS * 3];
+  __shared__ __attribute__((aligned(16))) u16 fring[RROWS * RFW * 8];
+  __shared__ __attribute__((aligned(16))) u16 ring1[RROWS * RPX * FPS];
+  __shared__ __attribute__((aligned(16))) u16 ring2[RROWS * RPX * FPS];
+  __shared__ __attribute__((aligned(16))) u16 ring3[RROWS * RPX * FPS];
   __shared__ double red[RNW * 2];
   const int item = blockIdx.x;
   const int strip = item % nstrip, rest = item / nstrip;
@@ -1331,16 +1366,16 @@ extern "C" __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __
   double v = 0.0, fl = 0.0;
   const float* bias = w + CL_TAPS * CL_C * CL_C;  // + layer * CL_LAYER_PARAMS
   if (layer == 0)
-    rm_march<0>(wl1, bias, fring, ring1, j, lane, x0, ya, nrows, W, H, od, raw, v, fl, mass, D, cur);
+    rm_march<0>(wl1, bias, fring, ring1, j, lane, x0, ya, nrows, W, H, od, v, fl, mass, D, cur);
   else if (layer == 1)
-    rm_march<1>(wf + CL_PACKED, bias + CL_LAYER_PARAMS, ring1, ring2, j, lane, x0, ya, nrows, W, H, od, raw, v,
-                fl, mass, D, cur);
+    rm_march<1>(wf + CL_PACKED, bias + CL_LAYER_PARAMS, ring1, ring2, j, lane, x0, ya, nrows, W, H, od, v, fl,
+                mass, D, cur);
   else if (layer == 2)
     rm_march<2>(wf + 2 * CL_PACKED, bias + 2 * CL_LAYER_PARAMS, ring2, ring3, j, lane, x0, ya, nrows, W, H, od,
-                raw, v, fl, mass, D, cur);
+                v, fl, mass, D, cur);
   else
     rm_march<3>(wf + 3 * CL_PACKED, bias + 3 * CL_LAYER_PARAMS, ring3, fring, j, lane, x0, ya, nrows, W, H, od,
-                raw, v, fl, mass, D, cur);
+                v, fl, mass, D, cur);
   if (mass) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -1506,14 +1541,14 @@ hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, co
   }
   if (form == 2) {
     // row march: strips of RS columns, spans of rows chosen so that the work-groups (one
-    // per CU at a time) finish in the fewest march steps, counting 12 warm-up steps each
+    // per CU at a time) finish in the fewest march steps, (span + 21) / 2 each
     const int nstrip = (W + RS - 1) / RS;
     int best = H, best_cost = -1;
     for (int span = 16; span < H + 16; span += 16) {
       const int sp = span < H ? span : H;
       const long items = (long)ndom * nstrip * ((H + sp - 1) / sp);
       const long rounds = (items + cus[dev] - 1) / cus[dev];
-      const long cost = rounds * (sp + 12);
+      const long cost = rounds * ((sp + 21) / 2);
       if (best_cost < 0 || cost < best_cost) {
         best_cost = cost;
         best = sp;
