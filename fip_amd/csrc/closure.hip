@@ -1114,10 +1114,10 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
 // every row of every layer is computed once: no vertical halo is recomputed, and the
 // horizontal halo is 3 columns per side at layer 0 (the tile forms recompute a 3-cell
 // halo on all four sides). March step k loads the features of rows ya - 4 + 2k and
-// ya - 3 + 2k, and layer L computes rows ya - (8 + 4 L) + 2k and the row after it, so
-// every input row was written in an earlier step; one barrier per step. 16 wavefronts:
-// wavefront 4 L + j runs layer L on pixel tile j (16 columns), and the four wavefronts on
-// a SIMD are one of each layer. Each keeps the A fragments of its layer in registers for
+// ya - 3 + 2k, and layer L computes rows ya - M_L + 2k and the row after it (M_L = 8,
+// 14, 18, 24), so every input row was written in an earlier half-step; two barriers per
+// step. 16 wavefronts: wavefront 4 L + j runs layer L on pixel tile j (16 columns), and
+// the four wavefronts on a SIMD are one of each layer. Each keeps the A fragments of its layer in registers for
 // the whole march, and each B fragment it reads feeds both of its rows. Every output sums
 // the same fragments in the same order as cl_fused: the same bits. The lanes that load
 // the features also add the mass terms of their rows; cl_correct(_ens) applies the
@@ -1209,14 +1209,14 @@ struct RmRow {
 // One wavefront of layer L (0..3) on pixel tile j over the march steps: reads its input
 // ring `in`, writes its output ring `out` (layer 3: the outputs o; it also loads the
 // features of the next rows, writes them to `out`, the feature ring, and adds their mass
-// terms). Layer L computes rows ya - (8 + 4 L) + 2k and the next at step k.
+// terms). Layer L computes rows ya - M + 2k and the next at step k (M = 8, 14, 18, 24).
 template <int L>
 __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u16* in, u16* out, int j, int lane,
                                          int x0, int ya, int nrows, int W, int H, float* od, double& v, double& fl,
                                          bool mass, const ClDom& D, int cur) {
   constexpr int NC = L == 0 ? 3 : CL_TAPS, MT = L == 3 ? 1 : 2;
   constexpr int RW = RS + 6 - 2 * L;       // region width
-  constexpr int M = 8 + 4 * L;             // first row of step k: ya - M + 2k
+  constexpr int M = L == 0 ? 8 : L == 1 ? 14 : L == 2 ? 18 : 24;  // pair k: rows ya - M + 2k, +1
   constexpr int HALO = 3 - L;              // rows the next layer needs: [ya - HALO, yb + HALO)
   const int col = lane & 15, grp = lane >> 4;
   const int c = 16 * j + col;
@@ -1253,7 +1253,7 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
     load_row(nr[0], ya - 4);
     load_row(nr[1], ya - 3);
   }
-  const int K = (nrows + 21) / 2;          // the last pair of layer 3 holds row yb - 1
+  const int K = (nrows + 25) / 2;          // the last pair of layer 3 holds row yb - 1
   const u16* inl = in + (cc + 1) * FPS + 8 * grp;  // layers 1..3: the lane's B-fragment base
   int l0off[3][RROWS];                               // layer 0: see rm_mfma2_l0
   if constexpr (L == 0) {
@@ -1266,69 +1266,101 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
     }
   }
   const bool tap8 = grp == 0;  // chunk 2 holds tap 8 in lane group 0 only
-  const auto step = [&](auto phase, int k) {
-    constexpr int PH = decltype(phase)::value;       // k & 3
-    constexpr int S0 = rm_slot(2 * PH - M);          // slot of this layer's row y = ya - M + 2k
-    if constexpr (L == 3) {
-      const int yF = ya - 4 + 2 * k;                 // slot (yF - ya) & 7 = rm_slot(2 PH - 4)
-      if (loader) {
+  // pair k of this layer: rows ya - M + 2k and the next; it is computed if it meets the
+  // rows the next layer needs
+  const auto active = [&](int k) { return 2 * k + 1 - M >= -HALO && 2 * k - M < nrows + HALO; };
+  f32x4 acc0[MT], acc1[MT];  // a pair between its matrix instructions and its epilogue
+  const auto mfma_pair = [&](auto slot) {
+    constexpr int S = decltype(slot)::value;
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
-          const int yr = yF + r;
-          if (2 * k + r < nfeat) {  // features of row yr (the formulas of feat_pixel)
-            const RmRow& w = nr[r];
-            uint4 pk = make_uint4(0, 0, 0, 0);
-            if (fxin && yr >= 0 && yr < H) {
-              const float f0v = (w.h + w.b - D.zref) * 0.25f;
-              const float f3v = (w.b - D.zref) * 0.25f;
-              const float f4v = w.bs * 2.0f;
-              const float f5v = w.k * (1.0f / 30.0f);
-              pk.x = (uint32_t)f2bf(f0v) | ((uint32_t)f2bf(w.qx) << 16);
-              pk.y = (uint32_t)f2bf(w.qy) | ((uint32_t)f2bf(f3v) << 16);
-              pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
-            }
-            *(uint4*)(out + (rm_slot(2 * PH - 4 + r) * RFW + fc) * 8) = pk;  // the feature ring
-            if (fmass && yr >= ya && yr < ya + nrows) {  // mass terms of an owned cell
-              const bool rows = yr >= 2 && yr <= H - 3, cols = fx >= 2 && fx <= W - 3;
-              if (rows && cols) v += (double)w.h;
-              if (rows && fx == W - 3) fl += (double)w.qx;
-              if (rows && fx == 1) fl -= (double)w.qx;
-              if (cols && yr == H - 3) fl += (double)w.qy;
-              if (cols && yr == 1) fl -= (double)w.qy;
-            }
-          }
+    for (int m = 0; m < MT; m++) acc0[m] = acc1[m] = bias[m];
+    if constexpr (L == 0)
+      rm_mfma2_l0<S>(in, l0off, tap8, a, acc0, acc1);
+    else
+      rm_mfma2<MT, S>(inl, a, acc0, acc1);
+  };
+  const auto epi_pair = [&](auto slot, int k) {
+    constexpr int S = decltype(slot)::value;
+    const int y = ya - M + 2 * k;
+    if constexpr (L < 3) {
+      rm_store<RW, S>(out, y, c, x0 + c - HALO, W, H, acc0, grp);
+      rm_store<RW, rm_slot(S + 1)>(out, y + 1, c, x0 + c - HALO, W, H, acc1, grp);
+    } else {  // outputs 0 and 1 (lane group 0) of rows y and y + 1 that the span owns
+      const int x = x0 + c;
+      if (grp == 0 && c < RS && x < W) {
+        if (y >= ya && y < ya + nrows) {
+          od[((long)y * W + x) * 2] = acc0[0][0];
+          od[((long)y * W + x) * 2 + 1] = acc0[0][1];
         }
-        if (2 * k + 2 < nfeat) {  // the next pair lands during this step
-          load_row(nr[0], yF + 2);
-          load_row(nr[1], yF + 3);
+        if (y + 1 >= ya && y + 1 < ya + nrows) {
+          od[((long)(y + 1) * W + x) * 2] = acc1[0][0];
+          od[((long)(y + 1) * W + x) * 2 + 1] = acc1[0][1];
         }
       }
     }
-    const int y = ya - M + 2 * k;
-    if (2 * k + 1 - M >= -HALO && 2 * k - M < nrows + HALO) {
-      f32x4 acc0[MT], acc1[MT];
+  };
+  // the loader lanes of layer 3: features of rows ya - 4 + 2k and the next into the
+  // feature ring, their mass terms, and the loads of the following pair
+  const auto feat_pair = [&](auto phase, int k) {
+    constexpr int PH = decltype(phase)::value;
+    const int yF = ya - 4 + 2 * k;                   // slot (yF - ya) & 7 = rm_slot(2 PH - 4)
+    if (!loader) return;
 #pragma unroll
-      for (int m = 0; m < MT; m++) acc0[m] = acc1[m] = bias[m];
-      if constexpr (L == 0)
-        rm_mfma2_l0<S0>(in, l0off, tap8, a, acc0, acc1);
-      else
-        rm_mfma2<MT, S0>(inl, a, acc0, acc1);
-      if constexpr (L < 3) {
-        rm_store<RW, S0>(out, y, c, x0 + c - HALO, W, H, acc0, grp);
-        rm_store<RW, rm_slot(S0 + 1)>(out, y + 1, c, x0 + c - HALO, W, H, acc1, grp);
-      } else {  // outputs 0 and 1 (lane group 0) of rows y and y + 1 that the span owns
-        const int x = x0 + c;
-        if (grp == 0 && c < RS && x < W) {
-          if (y >= ya && y < ya + nrows) {
-            od[((long)y * W + x) * 2] = acc0[0][0];
-            od[((long)y * W + x) * 2 + 1] = acc0[0][1];
-          }
-          if (y + 1 >= ya && y + 1 < ya + nrows) {
-            od[((long)(y + 1) * W + x) * 2] = acc1[0][0];
-            od[((long)(y + 1) * W + x) * 2 + 1] = acc1[0][1];
-          }
+    for (int r = 0; r < 2; r++) {
+      const int yr = yF + r;
+      if (2 * k + r < nfeat) {  // features of row yr (the formulas of feat_pixel)
+        const RmRow& w = nr[r];
+        uint4 pk = make_uint4(0, 0, 0, 0);
+        if (fxin && yr >= 0 && yr < H) {
+          const float f0v = (w.h + w.b - D.zref) * 0.25f;
+          const float f3v = (w.b - D.zref) * 0.25f;
+          const float f4v = w.bs * 2.0f;
+          const float f5v = w.k * (1.0f / 30.0f);
+          pk.x = (uint32_t)f2bf(f0v) | ((uint32_t)f2bf(w.qx) << 16);
+          pk.y = (uint32_t)f2bf(w.qy) | ((uint32_t)f2bf(f3v) << 16);
+          pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
+        }
+        *(uint4*)(out + (rm_slot(2 * PH - 4 + r) * RFW + fc) * 8) = pk;  // the feature ring
+        if (fmass && yr >= ya && yr < ya + nrows) {  // mass terms of an owned cell
+          const bool rows = yr >= 2 && yr <= H - 3, cols = fx >= 2 && fx <= W - 3;
+          if (rows && cols) v += (double)w.h;
+          if (rows && fx == W - 3) fl += (double)w.qx;
+          if (rows && fx == 1) fl -= (double)w.qx;
+          if (cols && yr == H - 3) fl += (double)w.qy;
+          if (cols && yr == 1) fl -= (double)w.qy;
         }
       }
+    }
+    if (2 * k + 2 < nfeat) {  // the next pair lands during the next half-step
+      load_row(nr[0], yF + 2);
+      load_row(nr[1], yF + 3);
+    }
+  };
+  // A march step is two halves, each closed by a barrier. Layers 1 and 3 issue the matrix
+  // instructions of pair k in the first half, layers 0 and 2 in the second; layer 1
+  // stores its pair in the second half, layers 0 and 2 theirs in the first half of the
+  // next step, layer 3 at once. So on every SIMD the matrix work of two wavefronts meets
+  // the epilogues of the other two. The row offsets M make every input row stored in an
+  // earlier half-step, and the 8-row rings hold each row until its last reader is done.
+  const auto step = [&](auto phase, int k) {
+    constexpr int PH = decltype(phase)::value;       // k & 3
+    using Cur = std::integral_constant<int, rm_slot(2 * PH - M)>;       // slot of pair k
+    using Prev = std::integral_constant<int, rm_slot(2 * PH - 2 - M)>;  // slot of pair k - 1
+    if constexpr (L == 1 || L == 3) {
+      if (active(k)) {
+        mfma_pair(Cur{});
+        if constexpr (L == 3) epi_pair(Cur{}, k);
+      }
+    } else {
+      if (k >= 1 && active(k - 1)) epi_pair(Prev{}, k - 1);
+    }
+    __syncthreads();
+    if constexpr (L == 0 || L == 2) {
+      if (active(k)) mfma_pair(Cur{});
+    } else if constexpr (L == 1) {
+      if (active(k)) epi_pair(Cur{}, k);
+    } else {
+      feat_pair(phase, k);
     }
     __syncthreads();
   };
@@ -1541,14 +1573,14 @@ hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, co
   }
   if (form == 2) {
     // row march: strips of RS columns, spans of rows chosen so that the work-groups (one
-    // per CU at a time) finish in the fewest march steps, (span + 21) / 2 each
+    // per CU at a time) finish in the fewest march steps, (span + 25) / 2 each
     const int nstrip = (W + RS - 1) / RS;
     int best = H, best_cost = -1;
     for (int span = 16; span < H + 16; span += 16) {
       const int sp = span < H ? span : H;
       const long items = (long)ndom * nstrip * ((H + sp - 1) / sp);
       const long rounds = (items + cus[dev] - 1) / cus[dev];
-      const long cost = rounds * ((sp + 21) / 2);
+      const long cost = rounds * ((sp + 25) / 2);
       if (best_cost < 0 || cost < best_cost) {
         best_cost = cost;
         best = sp;
