@@ -85,7 +85,13 @@ struct fip_ens {
   bool built = false;
   bool started = false;
   bool run_set = false;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // captured chunks: `steps` steps from parity `par`, then the flood-plain kernel (which
+  // tests iteration % 100 itself); most recently built last
+  struct Chunk {
+    int par, steps;
+    hipGraphExec_t exec;
+  };
+  std::vector<Chunk> graphs;
   bool graph_failed = false;
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -96,11 +102,8 @@ struct fip_ens {
 namespace {
 
 void destroy_graphs(fip_ens* e) {
-  for (auto& g : e->graph)
-    if (g) {
-      hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  for (auto& c : e->graphs) hipGraphExecDestroy(c.exec);
+  e->graphs.clear();
 }
 
 void free_tables(fip_ens* e) {
@@ -333,19 +336,36 @@ int one_step(fip_ens* e, int par) {
   return rc;
 }
 
-int build_graph(fip_ens* e, int par) {
+// A run uses at most three chunk lengths (up to the next multiple of kStepsPerGraph, whole
+// chunks, the rest), each at one or two parities.
+constexpr size_t kMaxGraphs = 6;
+
+// The graph of `steps` steps (1..kStepsPerGraph) from parity par, captured, instantiated
+// and uploaded on first use; nullptr when that failed.
+hipGraphExec_t chunk_graph(fip_ens* e, int par, int steps) {
+  for (auto& c : e->graphs)
+    if (c.par == par && c.steps == steps) return c.exec;
   hipGraph_t graph;
-  EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
   int rc = FIP_OK;
-  for (int k = 0; k < kStepsPerGraph && rc == FIP_OK; k++) rc = one_step(e, par ^ (k & 1));
+  for (int k = 0; k < steps && rc == FIP_OK; k++) rc = one_step(e, par ^ (k & 1));
   if (rc == FIP_OK) rc = launch_flood(e);
-  hipError_t er = hipStreamEndCapture(e->stream, &graph);
-  if (rc != FIP_OK) return rc;
-  if (er != hipSuccess) return fail(FIP_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(er));
-  er = hipGraphInstantiate(&e->graph[par], graph, nullptr, nullptr, 0);
+  const hipError_t er = hipStreamEndCapture(e->stream, &graph);
+  if (rc != FIP_OK || er != hipSuccess) return nullptr;
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
-  if (er != hipSuccess) return fail(FIP_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(er));
-  return FIP_OK;
+  if (ei != hipSuccess) return nullptr;
+  if (hipGraphUpload(exec, e->stream) != hipSuccess || hipStreamSynchronize(e->stream) != hipSuccess) {
+    hipGraphExecDestroy(exec);
+    return nullptr;
+  }
+  if (e->graphs.size() == kMaxGraphs) {
+    hipGraphExecDestroy(e->graphs.front().exec);
+    e->graphs.erase(e->graphs.begin());
+  }
+  e->graphs.push_back({par, steps, exec});
+  return exec;
 }
 
 int check_scn(fip_ens* e, int s) {
@@ -548,6 +568,29 @@ int fip_ens_tune(const fip_desc* descs, int n, const fip_bc* const* bcs, const i
   return FIP_OK;
 }
 
+int fip_ens_prepare(fip_ens* e, long long nsteps) {
+  if (!e) return fail(FIP_E_INVALID, "null argument");
+  if (!e->run_set) return fail(FIP_E_STATE, "call fip_ens_set_run first");
+  if (nsteps < 0) return fail(FIP_E_INVALID, "nsteps must not be negative");
+  EHIP(hipSetDevice(e->dev));
+  int rc;
+  if (!e->built && (rc = build(e))) return rc;
+  if (e->dense && e->items_wpb != e->wpb && (rc = plan_dense(e))) return rc;
+  if (e->timing || e->graph_failed) return FIP_OK;
+  // the chunks fip_ens_simulate cuts the steps into (chunk_graph returns the kept ones)
+  for (long long it = e->iter, left = nsteps; left > 0;) {
+    const long long k = std::min<long long>(kStepsPerGraph - (it % kStepsPerGraph), left);
+    if (!chunk_graph(e, (int)(it & 1), (int)k)) {
+      e->graph_failed = true;
+      hipGetLastError();
+      break;
+    }
+    it += k;
+    left -= k;
+  }
+  return FIP_OK;
+}
+
 int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps_done) {
   if (!e) return fail(FIP_E_INVALID, "null argument");
   if (steps_done) *steps_done = 0;
@@ -577,15 +620,16 @@ int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps
     const int par = (int)(e->iter & 1);
     const long long to_flood = kStepsPerGraph - (e->iter % kStepsPerGraph);
     const long long k = std::min<long long>(to_flood, nsteps - done);
-    const bool use_graph = k == kStepsPerGraph && !e->timing && !e->graph_failed;
-    if (use_graph && !e->graph[par]) {
-      if (build_graph(e, par) != FIP_OK) {
+    hipGraphExec_t gx = nullptr;
+    if (!e->timing && !e->graph_failed) {
+      gx = chunk_graph(e, par, (int)k);
+      if (!gx) {
         e->graph_failed = true;
         hipGetLastError();
       }
     }
-    if (use_graph && e->graph[par]) {
-      EHIP(hipGraphLaunch(e->graph[par], e->stream));
+    if (gx) {
+      EHIP(hipGraphLaunch(gx, e->stream));
       e->stats.stencil_launches += k;
     } else {
       for (long long i = 0; i < k; i++)

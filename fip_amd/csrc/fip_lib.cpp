@@ -112,6 +112,7 @@ struct fip_sim {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   fip_stats stats{};
   std::string arch;
+  int* pin = nullptr;  // pinned host copy of the control words read after every chunk
 };
 
 namespace {
@@ -213,6 +214,26 @@ int ctl_read(fip_sim* s, size_t off, T* v, size_t n = 1) {
 }
 
 #define CTL_OFF(f) offsetof(fip_ctl, f)
+
+// The control words fip_simulate reads after every chunk, from `iteration` through
+// `nsample_rows`, in one copy to pinned memory.
+struct CtlStatus {
+  int iteration, halt_req[2], nsample_rows;
+};
+constexpr size_t kStatusWords = (CTL_OFF(nsample_rows) - CTL_OFF(iteration)) / sizeof(int) + 1;
+static_assert(CTL_OFF(halt_req) == CTL_OFF(iteration) + sizeof(int), "control block layout");
+static_assert(kStatusWords <= 16, "control block layout");
+
+int ctl_status(fip_sim* s, CtlStatus* st) {
+  HIPTRY(hipMemcpyAsync(s->pin, (char*)s->g.ctl + CTL_OFF(iteration), kStatusWords * sizeof(int),
+                        hipMemcpyDeviceToHost, s->stream));
+  HIPTRY(hipStreamSynchronize(s->stream));
+  st->iteration = s->pin[0];
+  st->halt_req[0] = s->pin[1];
+  st->halt_req[1] = s->pin[2];
+  st->nsample_rows = s->pin[kStatusWords - 1];
+  return FIP_OK;
+}
 
 // Boundary kernels for one BC list: the counterpart of the string templates of
 // src/bc_kernel_gen.cpp:11-152, split in two launches.
@@ -553,6 +574,11 @@ hipGraphExec_t chunk_graph(fip_sim* s, int par, int steps, bool flood_first) {
   const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
   if (ei != hipSuccess) return nullptr;
+  // upload now, so that the chunk's first launch does not pay for it
+  if (hipGraphUpload(exec, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
+    hipGraphExecDestroy(exec);
+    return nullptr;
+  }
   if (s->graphs.size() == kMaxGraphs) {
     hipGraphExecDestroy(s->graphs.front().exec);
     s->graphs.erase(s->graphs.begin());
@@ -650,6 +676,8 @@ int fip_create(const fip_desc* d, fip_sim** out) {
   s->graph_failed = !fip_internal::graphs_enabled();
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(FIP_E_HIP, "hipStreamCreate failed"));
+  if (hipHostMalloc((void**)&s->pin, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    return cleanup(fail(FIP_E_HIP, "hipHostMalloc failed"));
 
   const int W = d->W, H = d->H;
   const int P = ((kMarginCols + W + kMarginCols) + 63) / 64 * 64;
@@ -794,6 +822,7 @@ int fip_destroy(fip_sim* s) {
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
   if (s->stream) hipStreamDestroy(s->stream);
+  if (s->pin) hipHostFree(s->pin);
   delete s;
   return FIP_OK;
 }
@@ -988,8 +1017,9 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
   HIPTRY(hipSetDevice(s->dev));
   int rc;
   // pending stop requests: a finished run stays finished; a handled dump is cleared
-  int req[2];
-  if ((rc = ctl_read(s, CTL_OFF(halt_req), req, 2))) return rc;
+  CtlStatus c0;
+  if ((rc = ctl_status(s, &c0))) return rc;
+  const int* req = c0.halt_req;
   if ((req[0] | req[1]) & FIP_HALT_FINISHED) {
     *event = FIP_EV_FINISHED;
     return FIP_OK;
@@ -1031,13 +1061,13 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
         if ((rc = one_step(s, (it0 + (int)i) & 1, i == 0 && flood_first))) return rc;
       if (!fused_flood(s) && (rc = launch_misc(s, 3, 1, 0))) return rc;
     }
-    int it1;
-    if ((rc = ctl_read(s, CTL_OFF(iteration), &it1))) return rc;
-    if ((rc = ctl_read(s, CTL_OFF(halt_req), req, 2))) return rc;
+    CtlStatus cs;
+    if ((rc = ctl_status(s, &cs))) return rc;
+    const int it1 = cs.iteration;
     s->stats.steps += it1 - it0;
     done += it1 - it0;
     s->host_iter = it1;
-    const int r = req[0] | req[1];
+    const int r = cs.halt_req[0] | cs.halt_req[1];
     if (r & FIP_HALT_DUMP) {
       *event = FIP_EV_DUMP;
       break;
@@ -1046,13 +1076,9 @@ int fip_simulate(fip_sim* s, long long nsteps, int* event, long long* steps_done
       *event = FIP_EV_FINISHED;
       break;
     }
-    if (s->npoints > 0) {
-      int rows = 0;
-      if ((rc = ctl_read(s, CTL_OFF(nsample_rows), &rows))) return rc;
-      if (rows > FIP_SAMPLE_LOG_ROWS - kStepsPerGraph - 1) {
-        *event = FIP_EV_SAMPLES;
-        break;
-      }
+    if (s->npoints > 0 && cs.nsample_rows > FIP_SAMPLE_LOG_ROWS - kStepsPerGraph - 1) {
+      *event = FIP_EV_SAMPLES;
+      break;
     }
   }
   if (steps_done) *steps_done = done;

@@ -30,7 +30,7 @@ CLOSURE_EXPORTS = ["fip_restrict", "fip_closure_create", "fip_closure_destroy", 
                    "fip_closure_set_kernel", "fip_mfma_probe"]
 CL_PARAMS = 4 * (9 * 32 * 32 + 32)
 ENS_EXPORTS = ["fip_ens_create", "fip_ens_destroy", "fip_ens_set_bcs", "fip_ens_set_run",
-               "fip_ens_simulate", "fip_ens_get_time", "fip_ens_download", "fip_ens_set_launch",
+               "fip_ens_simulate", "fip_ens_prepare", "fip_ens_get_time", "fip_ens_download", "fip_ens_set_launch",
                "fip_ens_tune",
                "fip_ens_set_timing", "fip_ens_get_stats"]
 
@@ -107,6 +107,7 @@ def lib():
         L.fip_ens_set_bcs.argtypes = [vp, C.c_int, C.POINTER(FipBC), C.c_int]
         L.fip_ens_set_run.argtypes = [vp, C.POINTER(C.c_double)]
         L.fip_ens_simulate.argtypes = [vp, C.c_longlong, ip, C.POINTER(C.c_longlong)]
+        L.fip_ens_prepare.argtypes = [vp, C.c_longlong]
         L.fip_ens_get_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_float), ip]
         L.fip_ens_download.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.fip_ens_set_launch.argtypes = [vp, C.c_int, C.c_int]
@@ -227,6 +228,10 @@ class Ensemble:
         act, done = C.c_int(), C.c_longlong()
         _check(lib().fip_ens_simulate(self.e, int(nsteps), C.byref(act), C.byref(done)))
         return act.value, done.value
+
+    def prepare(self, nsteps):
+        """Capture the graphs of the next simulate(nsteps) call (runs nothing)."""
+        _check(lib().fip_ens_prepare(self.e, int(nsteps)))
 
     def time(self, s):
         t, dt, it = C.c_double(), C.c_float(), C.c_int()

@@ -32,6 +32,12 @@ int fip_ens_set_run(fip_ens* e, const double* target_durations);
 /* Runs up to nsteps ensemble steps, or until every scenario reached its target.
  * *active receives the number of scenarios still running. */
 int fip_ens_simulate(fip_ens* e, long long nsteps, int* active, long long* steps_done);
+/* Captures, without running anything, the hipGraphs that the next fip_ens_simulate(e,
+ * nsteps) call launches: its steps are cut at multiples of 100 ensemble steps (each
+ * chunk ends with the flood-plain kernel, which tests every scenario's iteration itself),
+ * and each chunk of a length and parity is captured once and kept, as fip_prepare does
+ * for one simulation. */
+int fip_ens_prepare(fip_ens* e, long long nsteps);
 int fip_ens_get_time(fip_ens* e, int s, double* duration, float* dt, int* iteration);
 int fip_ens_download(fip_ens* e, int s, int field, int side, float* dst);
 /* Launch shape of the ensemble stencil: wavefronts per work-group (1..16; 0 = the
