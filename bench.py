@@ -6,16 +6,15 @@ no_data polygon, left type-q hydrograph, right open BC), started from a state wi
 of the cells wet. A step is one simulate() iteration: the stencil kernel plus the
 post-step boundary/control kernel (and the flood-plain kernel every 100 steps), driven by
 the device-resident loop as hipGraph chunks (cut at multiples of 100 iterations; the
-chunks of the timed steps are captured during the warm-up, so the timed region only
-launches graphs). Inputs are resident in HBM before timing.
+chunks of the timed steps are captured and uploaded after the warm-up by fip_prepare, so
+the timed region only launches graphs). Inputs are resident in HBM before timing.
 
 Prints one JSON line: value = W*H*steps / wall seconds (Gcell-updates/s) in dense mode,
 the roofline of the stencil kernel (algorithmic bytes per cell of the step: h, qx, qy, bed
 and kSt read, h, qx, qy written = 32 B with per-cell kSt, 28 B with uniform kSt; timed
 with HIP events on the simulation's stream; the bytes this build moves per cell, 29 B with
 kSt stored as a one-byte class index, are reported beside it), the masked-mode effective
-rate, the exact-variant
-rate, the batched ensemble of 64 scenarios of uneven sizes (launch shape from the
+rate, the exact-variant rate, the batched ensemble of 64 scenarios of uneven sizes (launch shape from the
 ensemble autotuner), the learned subgrid closure (training on solver-produced pairs,
 then learned coarse vs plain fine ensembles on held-out scenarios: speed and depth RMSE,
 max-depth error, wet-extent IoU) and the CPU oracle on a bounded sample.
