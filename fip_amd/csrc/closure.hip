@@ -1235,7 +1235,7 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
   const int fx = x0 + fc - 4;
   const bool fxin = fx >= 0 && fx < W;
   const bool fmass = mass && loader && fc >= 4 && fc < RS + 4 && fx < W;  // a column of the strip
-  RmRow nr[2];
+  RmRow nr[2][2];  // the features of pairs k and k + 1 (loads issued two pairs ahead)
   const auto load_row = [&](RmRow& r, int yy) {
     r = RmRow{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (loader && fxin && yy >= 0 && yy < H) {
@@ -1250,8 +1250,11 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
   };
   const int nfeat = nrows + 8;             // feature rows [ya - 4, yb + 4)
   if (L == 3) {
-    load_row(nr[0], ya - 4);
-    load_row(nr[1], ya - 3);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      load_row(nr[q][0], ya - 4 + 2 * q);
+      load_row(nr[q][1], ya - 3 + 2 * q);
+    }
   }
   const int K = (nrows + 25) / 2;          // the last pair of layer 3 holds row yb - 1
   const u16* inl = in + (cc + 1) * FPS + 8 * grp;  // layers 1..3: the lane's B-fragment base
@@ -1300,7 +1303,7 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
     }
   };
   // the loader lanes of layer 3: features of rows ya - 4 + 2k and the next into the
-  // feature ring, their mass terms, and the loads of the following pair
+  // feature ring, their mass terms, and the loads of pair k + 2
   const auto feat_pair = [&](auto phase, int k) {
     constexpr int PH = decltype(phase)::value;
     const int yF = ya - 4 + 2 * k;                   // slot (yF - ya) & 7 = rm_slot(2 PH - 4)
@@ -1309,7 +1312,7 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
     for (int r = 0; r < 2; r++) {
       const int yr = yF + r;
       if (2 * k + r < nfeat) {  // features of row yr (the formulas of feat_pixel)
-        const RmRow& w = nr[r];
+        const RmRow& w = nr[PH & 1][r];
         uint4 pk = make_uint4(0, 0, 0, 0);
         if (fxin && yr >= 0 && yr < H) {
           const float f0v = (w.h + w.b - D.zref) * 0.25f;
@@ -1331,9 +1334,9 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
         }
       }
     }
-    if (2 * k + 2 < nfeat) {  // the next pair lands during the next half-step
-      load_row(nr[0], yF + 2);
-      load_row(nr[1], yF + 3);
+    if (2 * k + 4 < nfeat) {  // pair k + 2: HBM latency hides behind two march steps
+      load_row(nr[PH & 1][0], yF + 4);
+      load_row(nr[PH & 1][1], yF + 5);
     }
   };
   // A march step is two halves, each closed by a barrier. Layers 1 and 3 issue the matrix
