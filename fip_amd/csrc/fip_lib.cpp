@@ -25,6 +25,8 @@ hipError_t fip_launch_stencil_fast(const fip_grid& g, int par, int masked, int b
 hipError_t fip_stencil_dense_waves_per_cu_exact(int kvar, int wpb, int* waves);
 hipError_t fip_stencil_dense_waves_per_cu_fast(int kvar, int wpb, int* waves);
 hipError_t fip_launch_misc(int which, const fip_grid& g, int arg, int blocks, hipStream_t st);
+hipError_t fip_launch_check_rcp(unsigned lo, unsigned n, unsigned long long* nbad, unsigned* first,
+                                hipStream_t st);
 
 namespace {
 
@@ -1294,6 +1296,35 @@ int fip_sync(fip_sim* s) {
   if (!s) return fail(FIP_E_INVALID, "null argument");
   HIPTRY(hipSetDevice(s->dev));
   HIPTRY(hipStreamSynchronize(s->stream));
+  return FIP_OK;
+}
+
+int fip_check_rcp(int device, float lo, float hi, unsigned long long* nbad, float* first_bad) {
+  if (!nbad || !first_bad) return fail(FIP_E_INVALID, "null argument");
+  if (!(lo > 0.0f) || !(hi >= lo) || !(hi <= 0x1p64f))
+    return fail(FIP_E_INVALID, "the range must satisfy 0 < lo <= hi <= 2^64");
+  HIPTRY(hipSetDevice(device));
+  unsigned long long* dn = nullptr;
+  unsigned* df = nullptr;
+  HIPTRY(hipMalloc(&dn, sizeof(*dn)));
+  hipError_t e = hipMalloc(&df, sizeof(*df));
+  unsigned a, bh;
+  std::memcpy(&a, &lo, 4);
+  std::memcpy(&bh, &hi, 4);
+  const unsigned n = bh - a + 1u;
+  const unsigned none = 0xffffffffu;
+  if (e == hipSuccess) e = hipMemset(dn, 0, sizeof(*dn));
+  if (e == hipSuccess) e = hipMemcpy(df, &none, sizeof(none), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = fip_launch_check_rcp(a, n, dn, df, nullptr);
+  unsigned f = none;
+  if (e == hipSuccess) e = hipMemcpy(nbad, dn, sizeof(*dn), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&f, df, sizeof(f), hipMemcpyDeviceToHost);
+  (void)hipFree(dn);
+  if (df) (void)hipFree(df);
+  if (e != hipSuccess) return fail(FIP_E_HIP, std::string("fip_check_rcp: ") + hipGetErrorString(e));
+  float fb = 0.0f;
+  if (f != none) std::memcpy(&fb, &f, 4);
+  *first_bad = fb;
   return FIP_OK;
 }
 

@@ -43,6 +43,29 @@ extern "C" __global__ void fip_k_floodplain(fip_grid g, int check) {
     }
 }
 
+// Self-check of fip_rcp_rn: every binary32 b with bits in [lo, lo + n) against the IEEE
+// division 1.0f / b; counts the differences and keeps the smallest such bit pattern.
+extern "C" __global__ void fip_k_check_rcp(unsigned lo, unsigned n, unsigned long long* nbad, unsigned* first) {
+  unsigned long long bad = 0;
+  unsigned lowest = 0xffffffffu;
+  for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const float b = fip_u2f(lo + k);
+    if (fip_f2u(fip_rcp_rn(b)) != fip_f2u(1.0f / b)) {
+      bad++;
+      lowest = lowest < lo + k ? lowest : lo + k;
+    }
+  }
+  if (bad) {
+    atomicAdd(nbad, bad);
+    atomicMin(first, lowest);
+  }
+}
+
+hipError_t fip_launch_check_rcp(unsigned lo, unsigned n, unsigned long long* nbad, unsigned* first, hipStream_t st) {
+  hipLaunchKernelGGL(fip_k_check_rcp, dim3(4096), dim3(256), 0, st, lo, n, nbad, first);
+  return hipGetLastError();
+}
+
 hipError_t fip_launch_misc(int which, const fip_grid& g, int arg, int blocks, hipStream_t st) {
   switch (which) {
     case 0: hipLaunchKernelGGL(fip_k_control, dim3(1), dim3(64), 0, st, g.ctl); break;

@@ -21,7 +21,7 @@ EXPORTS = ["fip_last_error", "fip_create", "fip_destroy", "fip_bc_generate", "fi
            "fip_floodplain", "fip_set_run", "fip_simulate", "fip_prepare", "fip_read_samples", "fip_get_time",
            "fip_set_dt", "fip_download", "fip_upload", "fip_tiles", "fip_get_masks",
            "fip_get_cfl", "fip_set_launch", "fip_tune", "fip_set_timing", "fip_get_stats",
-           "fip_sync"]
+           "fip_sync", "fip_check_rcp"]
 CLOSURE_EXPORTS = ["fip_restrict", "fip_closure_create", "fip_closure_destroy", "fip_closure_get_params",
                    "fip_closure_set_params", "fip_closure_add_sample", "fip_closure_num_samples",
                    "fip_closure_train", "fip_closure_adam_step", "fip_closure_eval", "fip_closure_attach",
@@ -137,6 +137,7 @@ def lib():
         L.fip_closure_fused_outputs.argtypes = [vp, vp, fp, C.c_float, fp]
         L.fip_closure_set_kernel.argtypes = [vp, C.c_int]
         L.fip_mfma_probe.argtypes = [C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), fp]
+        L.fip_check_rcp.argtypes = [C.c_int, C.c_float, C.c_float, C.POINTER(C.c_ulonglong), fp]
         L.fip_tune.argtypes = [C.POINTER(FipDesc), C.POINTER(FipBC), C.c_int, C.c_int,
                                C.POINTER(FipTune)]
         L.fip_ens_set_timing.argtypes = [vp, C.c_int]
@@ -465,6 +466,14 @@ def _f32p(a):
 def restrict(fine: "Sim", coarse: "Sim", factor: int = 4):
     """fip_restrict: the fine simulation's current state onto the coarse one."""
     _check(lib().fip_restrict(fine.s, coarse.s, int(factor)))
+
+
+def check_rcp(lo, hi, device=0):
+    """fip_check_rcp: (number of binary32 b in [lo, hi] where the exact variant's face-depth
+    reciprocal differs from the IEEE division, the smallest such b or 0)."""
+    n, f = C.c_ulonglong(), C.c_float()
+    _check(lib().fip_check_rcp(int(device), float(lo), float(hi), C.byref(n), C.byref(f)))
+    return n.value, f.value
 
 
 def mfma_probe(A, B, device=0):

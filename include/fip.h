@@ -197,6 +197,12 @@ int fip_tune(const fip_desc* desc, const fip_bc* bcs, int nbc, int steps, fip_tu
 int fip_set_timing(fip_sim* sim, int enable);
 int fip_get_stats(fip_sim* sim, fip_stats* out);
 int fip_sync(fip_sim* sim);
+/* Self-check (no counterpart in the reference): the exact variant forms the reciprocal of
+ * a face depth, 1/max(h, DH) (src/kernels.cu:89,116), as the correctly rounded RN(1/b)
+ * from the hardware reciprocal and two Newton steps. This runs that code on every
+ * binary32 b in [lo, hi] (0 < lo <= hi <= 2^64) on the given device against the IEEE
+ * division and returns how many differ and the smallest that does (0 if none). */
+int fip_check_rcp(int device, float lo, float hi, unsigned long long* nbad, float* first_bad);
 
 #ifdef __cplusplus
 }

@@ -270,3 +270,12 @@ def test_sampling_and_dumps():
     for (_, a), (_, b) in zip(got_dumps, dumps):
         for k in a:
             assert same_bits(a[k], b[k])
+
+
+def test_face_depth_reciprocal_is_correctly_rounded():
+    # the exact variant's reciprocal of a face depth (hardware reciprocal and two Newton
+    # steps) equals the IEEE division on every binary32 b in [DH, 2^64]: all face depths
+    # of a state without infinities (1/max(h, DH), src/kernels.cu:89,116)
+    from fip_amd.sim import check_rcp
+    nbad, first = check_rcp(np.float32(0.01), np.float32(2.0 ** 64))
+    assert nbad == 0, (nbad, first)

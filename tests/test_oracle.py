@@ -44,8 +44,9 @@ def test_pow233_correctly_rounded_on_sweep():
 
 def test_pow233_exhaustive_on_the_friction_domain():
     # every binary32 x in [0.01, 100] (the reciprocal of a face depth of 0.01 m to 100 m):
-    # correctly rounded except where x^2.33f lies within 2^-48 relative of a rounding
-    # midpoint, checked for each miss with 60-digit arithmetic
+    # correctly rounded except where x^2.33f lies within 2^-50 relative of a rounding
+    # midpoint (the error bound of the binary64 evaluation), checked for each miss with
+    # 60-digit arithmetic
     from decimal import Decimal, getcontext
     from oracle_lib import pow233_sweep
     getcontext().prec = 60
@@ -58,12 +59,13 @@ def test_pow233_exhaustive_on_the_friction_domain():
         got = np.float32(pow233(float(x)))
         other = np.nextafter(got, np.float32(np.inf) if Decimal(float(got)) < exact else np.float32(0))
         mid = (Decimal(float(got)) + Decimal(float(other))) / 2
-        assert abs(exact - mid) / exact < Decimal(2) ** -48, (x, exact, got)
+        assert abs(exact - mid) / exact < Decimal(2) ** -50, (x, exact, got)
 
 
 def test_pow233_tables_are_rounded_values():
-    # c_i = 128 / (129 + 2 i) to 28 significant bits (nearest), L_i = -ln c_i and
-    # T_j = 2^(j / 64) correctly rounded to binary64, as the header states
+    # c_i = 128 / (129 + 2 i) to 28 significant bits (nearest); M_i = c_i^-p, E[e] = 2^(e p)
+    # for e = -65..54 and the series coefficients b_k = binomial(p, k) correctly rounded to
+    # binary64, p = 2.33f, as the header states
     import re
     from decimal import Decimal, getcontext
     from fractions import Fraction
@@ -72,11 +74,14 @@ def test_pow233_tables_are_rounded_values():
                             "fip_amd", "csrc", "fip_detmath.h")).read()
 
     def table(name):
-        body = re.search(name + r"\[64\] = \{(.*?)\};", src, re.S).group(1)
-        return [float.fromhex(v) for v in re.findall(r"0x[0-9a-fp.+-]+", body)]
+        body = re.search(name + r"\[[^]]*\] = \{(.*?)\};", src, re.S).group(1)
+        return [float.fromhex(v) for v in re.findall(r"-?0x[0-9a-fp.+-]+", body)]
 
     def rn(x, bits):
         f = Fraction(x)
+        sign = 1
+        if f < 0:
+            f, sign = -f, -1
         e = 0
         while f >= 2:
             f /= 2
@@ -88,14 +93,29 @@ def test_pow233_tables_are_rounded_values():
         k = int(s)
         if s - k > Fraction(1, 2) or (s - k == Fraction(1, 2) and k % 2):
             k += 1
-        return float(Fraction(k, 1 << (bits - 1)) * Fraction(2) ** e)
+        return sign * float(Fraction(k, 1 << (bits - 1)) * Fraction(2) ** e)
 
-    c, lt, tt = table("kFipPowC"), table("kFipPowL"), table("kFipPowT")
-    assert len(c) == len(lt) == len(tt) == 64
+    p = Decimal(float(np.float32(2.33)))
+    c, m, et = table("kFipPowC"), table("kFipPowM"), table("kFipPowE")
+    assert len(c) == len(m) == 64 and len(et) == 120
     for i in range(64):
         assert c[i] == rn(Fraction(128, 129 + 2 * i), 28), i
-        assert lt[i] == rn(-(Decimal(c[i]).ln()), 53), i
-        assert tt[i] == rn(Decimal(2) ** (Decimal(i) / 64), 53), i
+        assert m[i] == rn((-(p * Decimal(c[i]).ln())).exp(), 53), i
+    for k, e in enumerate(range(-65, 55)):
+        assert et[k] == rn((Decimal(e) * p * Decimal(2).ln()).exp(), 53), e
+    # the series coefficients b6..b1 in the Horner steps of fip_pow233_tab
+    body = src[src.index("FIP_HD float fip_pow233_tab"):]
+    coef = {int(k): float.fromhex(v.replace(" ", "")) * (-1 if sgn == "-" else 1)
+            for v, sgn, k in [(m_.group(2), m_.group(1), m_.group(3)) for m_ in
+                              re.finditer(r"([-+]) (0x[0-9a-fp.+-]+);\s*// b(\d)", body)]}
+    coef[6] = float.fromhex(re.search(r"double s = (-?0x[0-9a-fp.+-]+);\s*// b6", body).group(1))
+    pf = Fraction(float(np.float32(2.33)))
+    for k in range(1, 7):
+        num, den = Fraction(1), 1
+        for j in range(k):
+            num *= pf - j
+            den *= j + 1
+        assert coef[k] == rn(num / den, 53), k
 
 
 def test_pow233_special_values():
