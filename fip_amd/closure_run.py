@@ -106,7 +106,9 @@ def generate_samples(cl, scenarios, duration, sample_every=60.0, variant="fast",
     from .sim import Sim, restrict
     per = int(round(sample_every / DT_FINE)) - F
     t0 = time.perf_counter()
-    for s in scenarios:
+    for k, s in enumerate(scenarios):
+        if k and k % 32 == 0:
+            log(f"samples: {cl.num_samples()} after {k} scenarios, {time.perf_counter() - t0:.1f} s")
         fine = fine_case(s, W=W, H=W, duration=duration)
         coarse, bstd, zref = coarse_of(fine)
         f = Sim(fine, mode="dense", variant=variant)
@@ -190,12 +192,13 @@ def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1
                lr=1e-3, W=1024, eval_W=2048, log=print):
     """Generate samples (fine grids of W^2), train, evaluate on held-out scenarios with
     fine grids of eval_W^2 (the network is convolutional, so it applies to any size). The
-    held-out samples come first in the store, the training samples after them."""
+    held-out samples (four per held-out scenario, one a minute) come first in the store, the
+    training samples (one every `sample_every` seconds of each training run) after them."""
     from .sim import Closure
     Wc = W // F
     nsamp = n_train * int(duration // sample_every) + n_test * 4
     cl = Closure(Wc, Wc, max_samples=nsamp, seed=11)
-    generate_samples(cl, range(1000, 1000 + n_test), 4 * sample_every, sample_every, W=W, log=log)
+    generate_samples(cl, range(1000, 1000 + n_test), 240.0, 60.0, W=W, log=log)
     ntest = cl.num_samples()
     generate_samples(cl, range(n_train), duration, sample_every, W=W, log=log)
     plain0, learned0 = cl.eval(0, ntest)
