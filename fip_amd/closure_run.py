@@ -267,6 +267,36 @@ def time_training(steps=200, scenarios=2, W=1024, duration=600.0, batch=32, patc
     return out
 
 
+def drift(scenarios=range(1000, 1004), times=(300, 900, 1800, 3600, 7200, 10800), W=2048, log=print):
+    """Plain coarse against the restricted plain fine run of the same held-out scenarios
+    over time, without any closure: depth RMSE, wet-extent IoU and the wet fractions of
+    both at each time. Returns a list of dicts."""
+    from . import metrics as co
+    from .sim import Sim
+    out = []
+    for s in scenarios:
+        fine = fine_case(s, W=W, H=W, duration=float(times[-1]))
+        coarse, _, _ = coarse_of(fine)
+        f = Sim(fine, mode="dense", variant="fast")
+        c = Sim(coarse, mode="dense", variant="fast")
+        f.set_run(1e12)
+        c.set_run(1e12)
+        for T in times:
+            f.simulate(int(round((T - f.time()[0]) / fine.dt)))
+            c.simulate(int(round((T - c.time()[0]) / coarse.dt)))
+            href = block_mean(f.field("h"))
+            hc = c.field("h")
+            rec = {"scenario": s, "t_s": T, "depth_rmse_m": round(co.depth_rmse(hc, href), 4),
+                   "wet_iou": round(co.wet_iou(hc, href), 4),
+                   "wet_fraction_fine": round(float((href > 0.01).mean()), 5),
+                   "wet_fraction_coarse": round(float((hc > 0.01).mean()), 5)}
+            log(str(rec))
+            out.append(rec)
+        f.close()
+        c.close()
+    return out
+
+
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="time the coarse ensemble step with and without the closure, "
@@ -278,8 +308,12 @@ if __name__ == "__main__":
     ap.add_argument("--warm", type=int, default=200)
     ap.add_argument("--form", type=int, default=None, help="fused kernel form (fip_closure_set_kernel)")
     ap.add_argument("--sat", type=float, default=1e30, help="saturation limit of the network outputs")
+    ap.add_argument("--drift", action="store_true",
+                    help="plain coarse against the restricted fine run over 3 h instead (no closure)")
     a = ap.parse_args()
-    if a.train_steps:
+    if a.drift:
+        drift()
+    elif a.train_steps:
         time_training(a.train_steps)
     else:
         time_ensemble_step(a.steps, a.scenarios, a.fine, a.warm, sat=a.sat, form=a.form)
