@@ -200,7 +200,10 @@ int build(fip_ens* e) {
 // wavefronts past the grid. All wavefronts of a work-group then march the same rows, so
 // the barriers of the march stay matched; neighbouring work-groups (which share halo
 // columns) land on one XCD. The span is the shortest (at least 16 rows, span + 2 a
-// multiple of the 6-row register ring) for which every work-group is resident at once.
+// multiple of the 6-row register ring) for which every work-group is resident at once in
+// the fast variant, and 16 rows in the exact one, whose cost per row depends on the
+// water: there the dispatcher balances short spans over many rounds (as in the single
+// domain's dense mode).
 int plan_dense(fip_ens* e) {
   int wpc = 0;
   hipError_t r = e->fast ? fip_ens_dense_waves_per_cu_fast(e->kvar ? 1 : 0, e->wpb, &wpc)
@@ -243,7 +246,7 @@ int plan_dense(fip_ens* e) {
   };
   const int hmax = *std::max_element(heights.begin(), heights.end());
   int span = 16;
-  while (waves(span) > slots && span < hmax) span += 6;
+  while (e->fast && waves(span) > slots && span < hmax) span += 6;
   ranges(span, rg);
   std::vector<int4> items;
   for (auto& k : rg) {
