@@ -129,14 +129,15 @@ def run_ensemble(variant, steps, warmup, nscn=64, modes=("dense", "masked")):
 FULL_RUN_SHAPE = {"tile_rows": 16, "waves_per_block": 2, "blocks_per_cu": 4}
 
 
-def full_run(variant, chunk=5000, max_steps=0, tuned=False):
+def full_run(variant, chunk=5000, max_steps=0, tuned=False, shape=None):
     """The 5667 x 6500 workload over its whole 12 h from the dry start in masked mode:
     steps, wall time, effective rate (W*H per step) and the active-tile fraction sampled
     every `chunk` steps. The launch shape is FULL_RUN_SHAPE, or (tuned) the autotuner's
     pick on the 30 %-wet state. Returns a dict for a JSON line."""
     from fip_amd.sim import Sim, tune
     from fip_amd.workloads import config1
-    best = tune(wet_state(config1()), mode="masked", variant=variant, steps=10) if tuned else dict(FULL_RUN_SHAPE)
+    best = (tune(wet_state(config1()), mode="masked", variant=variant, steps=10) if tuned
+            else dict(shape) if shape else dict(FULL_RUN_SHAPE))
     rows = best["tile_rows"]
     case = config1()
     valid = case.bed != np.float32(case.no_data)
@@ -163,7 +164,7 @@ def full_run(variant, chunk=5000, max_steps=0, tuned=False):
     out = {"workload": "plain large domain 5667x6500 dx=3 kSt_var no_data, left q hydrograph "
                        "0->600->0 m3/s over 12 h, right open, dry start, masked mode",
            "variant": variant, "shape": {k: best[k] for k in ("tile_rows", "waves_per_block", "blocks_per_cu")},
-           "shape_from": "fip_tune on the 30 %-wet state" if tuned else "whole-run sweep",
+           "shape_from": ("fip_tune on the 30 %-wet state" if tuned else "given" if shape else "whole-run sweep"),
            "simulated_s": round(s.time()[0], 2), "steps": steps,
            "wall_s": round(wall, 2), "ms_per_step": round(wall / steps * 1e3, 4),
            "gcells_effective": round(case.W * case.H * steps / wall / 1e9, 2),
@@ -383,6 +384,9 @@ def main():
                     help="full runs: take the launch shape from the autotuner on the 30%%-wet state")
     ap.add_argument("--full-run-steps", type=int, default=0,
                     help="stop the full run after this many steps (0: the whole 12 h)")
+    ap.add_argument("--full-run-shape", default=None,
+                    help="full run: tile rows, waves per work-group, work-groups per CU (e.g. 16,4,4) "
+                         "instead of the fip CLI's default")
     ap.add_argument("--ensemble-only", choices=["dense", "masked"],
                     help="run only the batched-ensemble leg in this mode and print its result")
     ap.add_argument("--cli-run", action="store_true",
@@ -402,8 +406,12 @@ def main():
         calibrate_copy()
         return
     if args.full_run:
-        print(json.dumps(full_run(args.variant, max_steps=args.full_run_steps, tuned=args.full_run_tuned)),
-              flush=True)
+        shape = None
+        if args.full_run_shape:
+            r, w, b = (int(v) for v in args.full_run_shape.split(","))
+            shape = {"tile_rows": r, "waves_per_block": w, "blocks_per_cu": b}
+        print(json.dumps(full_run(args.variant, max_steps=args.full_run_steps, tuned=args.full_run_tuned,
+                                  shape=shape)), flush=True)
         return
     if args.ensemble_full_run:
         print(json.dumps(ensemble_full_run(args.variant)), flush=True)
