@@ -49,11 +49,10 @@ struct ClDom {
 };
 #define CL_PACKED_L1 (2 * 3 * 64 * 8)   // bf16 elements of the 8-channel packing of layer 0
 
-// Network outputs o ([ndom][H][W][2]) of every cell of ndom domains of one size, and their
-// mass-balance sums into acc (2 per domain) when acc != NULL. form 2: row march,
-// 1: persistent tiles, 0: one work-group per tile.
+// Network outputs o ([ndom][H][W][2]) of every cell of ndom domains of one size. form 2:
+// row march, 1: persistent tiles, 0: one work-group per tile.
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                           const float* w, double* acc, float* o, hipStream_t st, int form = 1);
+                           const float* w, float* o, hipStream_t st, int form = 1);
 hipError_t cl_pack_l1_launch(const float* w, u16* wl1, hipStream_t st);
 hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
                           u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,
@@ -79,13 +78,15 @@ hipError_t cl_pack_launch(const float* w, int nlayers, u16* wf, u16* wb, hipStre
 hipError_t cl_loss_sum_launch(const double* part, int nb, float* out, hipStream_t st);
 hipError_t cl_adam_launch(float* w, float* m, float* v, const float* g, int nlayers, float lr, float b1,
                           float b2, float eps, float c1, float c2, hipStream_t st);
-// Correction of the current face discharges by o; saturated faces counted per work-group
-// of 256 columns of a row into part ((W + 255) / 256 * H slots per domain).
+// Correction of the current face discharges by o; saturated faces counted and the
+// mass-balance terms of the uncorrected state summed per work-group of 256 columns of a
+// row into part and mpart (2 doubles per slot; (W + 255) / 256 * H slots per domain),
+// which the mass check adds in a fixed order.
 hipError_t cl_correct_launch(fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms,
-                             unsigned* part, hipStream_t st);
-hipError_t cl_mass_check_launch(fip_grid g, int par, double* acc, ClMass* ms, double limit, const unsigned* part,
-                                int nparts, hipStream_t st);
+                             unsigned* part, double* mpart, hipStream_t st);
+hipError_t cl_mass_check_launch(fip_grid g, int par, const double* mpart, ClMass* ms, double limit,
+                                const unsigned* part, int nparts, hipStream_t st);
 hipError_t cl_correct_ens_launch(const fip_ens_scn* scn, int par, const float* o, float bound, float sat,
-                                 ClMass* ms, unsigned* part, int W, int H, int n, hipStream_t st);
-hipError_t cl_mass_check_ens_launch(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
-                                    const unsigned* part, int nparts, hipStream_t st);
+                                 ClMass* ms, unsigned* part, double* mpart, int W, int H, int n, hipStream_t st);
+hipError_t cl_mass_check_ens_launch(const fip_ens_scn* scn, int n, int par, const double* mpart, ClMass* ms,
+                                    double limit, const unsigned* part, int nparts, hipStream_t st);

@@ -461,44 +461,82 @@ extern "C" __global__ void cl_adam(float* w, float* m, float* v, const float* g,
 
 // The correction of cell (x, y)'s east and north faces by the network output o ([H][W][2]):
 // on interior faces q += tanh(o) * bound * |q| unless |o| > sat (counted, the face keeps
-// the plain step) or the domain fell back (off). Returns the saturated faces.
+// the plain step) or the domain fell back (off). Returns the saturated faces. It also
+// returns the cell's mass-balance terms of the uncorrected state (see cl_mass_check): v,
+// the depth of a cell of the box [2, W-3] x [2, H-3], and f, the discharges the step
+// computed through the box's faces (+ east and north, - west and south), each read by
+// the thread that corrects that face, before its correction.
 __device__ __forceinline__ unsigned correct_cell(const fip_grid& g, int cur, const float* o, int x, int y,
-                                                 float bound, float sat, bool off) {
+                                                 float bound, float sat, bool off, double& v, double& f) {
   const long p = (long)y * g.W + x, i = (long)y * g.P + x;
   unsigned nsat = 0;
+  v = 0.0;
+  f = 0.0;
+  if (x >= 2 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) v = (double)g.h[cur][i];
   if (x >= 1 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) {
     const float ox = o[p * 2];
     const float q = g.qx[cur][i];
+    if (x == g.W - 3) f += (double)q;
+    if (x == 1) f -= (double)q;
     if (fabsf(ox) > sat) nsat++;
     else if (!off) g.qx[cur][i] = q + tanhf(ox) * bound * fabsf(q);
   }
   if (x >= 2 && x <= g.W - 3 && y >= 1 && y <= g.H - 3) {
     const float oy = o[p * 2 + 1];
     const float q = g.qy[cur][i];
+    if (y == g.H - 3) f += (double)q;
+    if (y == 1) f -= (double)q;
     if (fabsf(oy) > sat) nsat++;
     else if (!off) g.qy[cur][i] = q + tanhf(oy) * bound * fabsf(q);
   }
   return nsat;
 }
 
-// The saturated faces of a work-group of 256 threads, written to its slot of `part`
-// (every thread must call it). The mass check adds the slots of a domain: one atomic per
-// work-group on the domain's single counter would serialise.
-__device__ __forceinline__ void block_count(unsigned* part, unsigned n) {
+// The saturated faces and the mass-balance terms of a work-group of 256 threads, written
+// to its slots of `part` and `mpart` (2 doubles) in a fixed order (every thread must call
+// it). The mass check adds the slots of a domain: one atomic per work-group on the
+// domain's single counter would serialise, and atomic sums of doubles would depend on
+// timing.
+__device__ __forceinline__ void block_count(unsigned* part, double* mpart, unsigned n, double v, double f) {
   __shared__ unsigned wsum[4];
+  __shared__ double wv[4], wf[4];
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m);
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+  for (int m = 32; m >= 1; m >>= 1) {
+    n += __shfl_xor(n, m);
+    v += __shfl_xor(v, m);
+    f += __shfl_xor(f, m);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wsum[threadIdx.x >> 6] = n;
+    wv[threadIdx.x >> 6] = v;
+    wf[threadIdx.x >> 6] = f;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) *part = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  if (threadIdx.x == 0) {
+    *part = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    mpart[0] = ((wv[0] + wv[1]) + wv[2]) + wv[3];
+    mpart[1] = ((wf[0] + wf[1]) + wf[2]) + wf[3];
+  }
 }
 
-// sum of nparts counts by one wavefront (lane 0 holds the result)
-__device__ __forceinline__ unsigned wave_sum_parts(const unsigned* part, int nparts) {
+// sum of nparts counts and mass-balance slots by one wavefront, in a fixed order (every
+// lane holds the result)
+__device__ __forceinline__ unsigned wave_sum_parts(const unsigned* part, const double* mpart, int nparts,
+                                                   double& v, double& f) {
   unsigned t = 0;
-  for (int i = threadIdx.x & 63; i < nparts; i += 64) t += part[i];
+  v = 0.0;
+  f = 0.0;
+  for (int i = threadIdx.x & 63; i < nparts; i += 64) {
+    t += part[i];
+    v += mpart[2 * i];
+    f += mpart[2 * i + 1];
+  }
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) t += __shfl_xor(t, m);
+  for (int m = 32; m >= 1; m >>= 1) {
+    t += __shfl_xor(t, m);
+    v += __shfl_xor(v, m);
+    f += __shfl_xor(f, m);
+  }
   return t;
 }
 
@@ -512,58 +550,32 @@ __device__ __forceinline__ unsigned wave_sum_parts(const unsigned* part, int npa
 // One work-group per 256 columns of a row; its count of saturated faces goes to
 // part[blockIdx.y * gridDim.x + blockIdx.x].
 extern "C" __global__ __launch_bounds__(256) void cl_correct(fip_grid g, int par, const float* o, float bound,
-                                                            float sat, ClMass* ms, unsigned* part) {
+                                                            float sat, ClMass* ms, unsigned* part, double* mpart) {
   if (g.ctl->halt_req[par ^ 1]) return;
   const int cur = par ^ 1;
   const bool off = ms->fallback != 0;
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   unsigned nsat = 0;
-  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o, x, y, bound, sat, off);
-  block_count(part + (long)blockIdx.y * gridDim.x + blockIdx.x, nsat);
+  double v = 0.0, f = 0.0;
+  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o, x, y, bound, sat, off, v, f);
+  const long slot = (long)blockIdx.y * gridDim.x + blockIdx.x;
+  block_count(part + slot, mpart + 2 * slot, nsat, v, f);
 }
 
 // Mass balance of a coarse domain without no_data cells, checked after each step before
 // its correction. The continuity update telescopes over the box of cells
 // [2, W-3] x [2, H-3], which no boundary condition writes, so over the step just taken
 //   V(t) - V(t - dt) = -dt dx (sum_j qx(W-3, j) - qx(1, j) + sum_i qy(i, H-3) - qy(i, 1))
-// with the face discharges the step computed (stored and not yet corrected). The fused
-// kernels add the box volume and the bracket into acc[2] (fp64); cl_mass_check compares
-// and sets the fallback flag when the relative error exceeds `limit`.
-__device__ __forceinline__ void mass_terms(const fip_grid& g, int cur, int x, int y, double& v, double& f) {
-  v = 0.0;
-  f = 0.0;
-  if (x >= 2 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) v = (double)g.h[cur][(long)y * g.P + x];
-  if (x == 0 && y >= 2 && y <= g.H - 3)
-    f = (double)g.qx[cur][(long)y * g.P + g.W - 3] - (double)g.qx[cur][(long)y * g.P + 1];
-  if (y == 0 && x >= 2 && x <= g.W - 3)
-    f += (double)g.qy[cur][(long)(g.H - 3) * g.P + x] - (double)g.qy[cur][(long)g.P + x];
-}
-
-// sh: 2 x blockDim.x doubles of LDS provided by the kernel (blockDim.x a power of 2)
-__device__ __forceinline__ void block_sum2(double v, double f, double* acc, double* sh) {
-  const int n = blockDim.x, t = threadIdx.x;
-  sh[t] = v;
-  sh[n + t] = f;
-  __syncthreads();
-  for (int w = n / 2; w > 0; w >>= 1) {
-    if (t < w) {
-      sh[t] += sh[t + w];
-      sh[n + t] += sh[n + t + w];
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    atomicAdd(&acc[0], sh[0]);
-    atomicAdd(&acc[1], sh[n]);
-  }
-}
-
-__device__ __forceinline__ void mass_check(const fip_grid& g, double* acc, ClMass& m, double limit) {
+// with the face discharges the step computed (stored and not yet corrected). cl_correct
+// adds the box's depths and the bracket per work-group (fp64, correct_cell); the check
+// adds those slots in a fixed order, compares, and sets the fallback flag when the
+// relative error exceeds `limit`.
+__device__ __forceinline__ void mass_check(const fip_grid& g, double vsum, double fsum, ClMass& m, double limit) {
   const double dx = g.dx;
-  const double vol = acc[0] * dx * dx;
+  const double vol = vsum * dx * dx;
   const double t = g.ctl->duration;
   if (m.valid) {
-    const double pred = m.v_prev - (t - m.t_prev) * dx * acc[1];
+    const double pred = m.v_prev - (t - m.t_prev) * dx * fsum;
     const double rel = fabs(vol - pred) / (vol > 1e-30 ? vol : 1e-30);
     m.last_rel = rel;
     if (rel > m.max_rel) m.max_rel = rel;
@@ -573,18 +585,18 @@ __device__ __forceinline__ void mass_check(const fip_grid& g, double* acc, ClMas
   m.t_prev = t;
   m.valid = 1;
   m.steps = m.steps + 1;
-  acc[0] = 0.0;
-  acc[1] = 0.0;
 }
 
-// One wavefront: adds the saturation counts of the step (nparts slots), then the check.
-extern "C" __global__ void cl_mass_check(fip_grid g, int par, double* acc, ClMass* ms, double limit,
+// One wavefront: adds the saturation counts and mass terms of the step (nparts slots),
+// then the check.
+extern "C" __global__ void cl_mass_check(fip_grid g, int par, const double* mpart, ClMass* ms, double limit,
                                          const unsigned* part, int nparts) {
   if (g.ctl->halt_req[par ^ 1]) return;
-  const unsigned t = wave_sum_parts(part, nparts);
+  double v, f;
+  const unsigned t = wave_sum_parts(part, mpart, nparts, v, f);
   if (threadIdx.x != 0) return;
   ms->nsat += t;
-  mass_check(g, acc, *ms, limit);
+  mass_check(g, v, f, *ms, limit);
 }
 
 // Ensemble forms of cl_correct / cl_mass_check: blockIdx.z (cl_mass_check_ens:
@@ -592,7 +604,7 @@ extern "C" __global__ void cl_mass_check(fip_grid g, int par, double* acc, ClMas
 // scenario.
 extern "C" __global__ __launch_bounds__(256) void cl_correct_ens(const fip_ens_scn* scn, int par, const float* o,
                                                                 float bound, float sat, ClMass* ms,
-                                                                unsigned* part) {
+                                                                unsigned* part, double* mpart) {
   const int s = blockIdx.z;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
@@ -600,20 +612,24 @@ extern "C" __global__ __launch_bounds__(256) void cl_correct_ens(const fip_ens_s
   const bool off = ms[s].fallback != 0;
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   unsigned nsat = 0;
-  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o + (long)s * g.W * g.H * 2, x, y, bound, sat, off);
+  double v = 0.0, f = 0.0;
+  if (x < g.W && y < g.H)
+    nsat = correct_cell(g, cur, o + (long)s * g.W * g.H * 2, x, y, bound, sat, off, v, f);
   const long nparts = (long)gridDim.x * gridDim.y;
-  block_count(part + s * nparts + (long)blockIdx.y * gridDim.x + blockIdx.x, nsat);
+  const long slot = s * nparts + (long)blockIdx.y * gridDim.x + blockIdx.x;
+  block_count(part + slot, mpart + 2 * slot, nsat, v, f);
 }
 
-extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms,
-                                             double limit, const unsigned* part, int nparts) {
+extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int par, const double* mpart,
+                                             ClMass* ms, double limit, const unsigned* part, int nparts) {
   const int s = blockIdx.x;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
-  const unsigned t = wave_sum_parts(part + (long)s * nparts, nparts);
+  double v, f;
+  const unsigned t = wave_sum_parts(part + (long)s * nparts, mpart + 2L * s * nparts, nparts, v, f);
   if (threadIdx.x != 0) return;
   ms[s].nsat += t;
-  mass_check(g, acc + 2 * s, ms[s], limit);
+  mass_check(g, v, f, ms[s], limit);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -725,8 +741,7 @@ __device__ __forceinline__ void fused_layer(const u16* in, u16* out, const u16* 
 }
 
 extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, int par, const u16* wf,
-                                                          const u16* wl1, const float* w, double* acc,
-                                                          float* o_out) {
+                                                          const u16* wl1, const float* w, float* o_out) {
   __shared__ __attribute__((aligned(16))) u16 f0[FR0 * FR0 * 8];
   __shared__ __attribute__((aligned(16))) u16 b1[FR1 * FR1 * FPS];
   __shared__ __attribute__((aligned(16))) u16 b2[FR2 * FR2 * FPS];
@@ -770,25 +785,13 @@ extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, 
   fused_layer<FR3, FT, 1, false>(b1, nullptr, wf + 3 * CL_PACKED, w + 3 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C,
                                  ox, oy, W, H, o2);
   __syncthreads();
-  // the outputs of the tile's cells into o (one [H][W][2] image per domain) and the
-  // mass-balance sums (acc != NULL) of the uncorrected state
+  // the outputs of the tile's cells into o (one [H][W][2] image per domain)
   float* od = o_out + (long)d * W * H * 2;
-  double v = 0.0, fl = 0.0;
   for (int e = threadIdx.x; e < FT * FT; e += blockDim.x) {
     const int x = ox + e % FT, y = oy + e / FT;
     if (x >= W || y >= H) continue;
     od[((long)y * W + x) * 2] = o2[e * 2];
     od[((long)y * W + x) * 2 + 1] = o2[e * 2 + 1];
-    if (acc) {
-      double mv, mf;
-      mass_terms(g, cur, x, y, mv, mf);
-      v += mv;
-      fl += mf;
-    }
-  }
-  if (acc) {
-    __syncthreads();
-    block_sum2(v, fl, acc + 2 * d, (double*)f0);  // the feature buffer (>= 2 x 512 doubles) is free by now
   }
 }
 
@@ -800,11 +803,7 @@ extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, 
 //     of all four layers are staged in LDS once per launch, not re-read from global memory
 //     by every wavefront of every tile;
 //   * the input cells of tile t+1 (bed, h, qx, qy, subgrid std, kSt) are loaded into
-//     registers while tile t runs layer 0, and written to the feature buffer after layer 1;
-//   * the mass terms take the discharges and depths of the tile's 16 x 16 cells from LDS
-//     (double-buffered); they are summed per thread over the work-group's tiles of one
-//     domain and reduced once per domain. Each box-boundary discharge is added by the
-//     thread of the cell that owns the face.
+//     registers while tile t runs layer 0, and written to the feature buffer after layer 1.
 // Like cl_fused it writes the network outputs; cl_correct(_ens) applies them.
 // Four barriers per tile. LDS: 52 KB of weights + 80 KB of activations (one work-group
 // per CU, 8 wavefronts).
@@ -972,9 +971,8 @@ __device__ __forceinline__ void fp_load(const FpDom& D, FpTile T, FpCells& c) {
   }
 }
 
-// features (the formulas of feat_pixel, as in cl_fused) into f0; the 16 x 16 core cells'
-// qx, qy and h into core
-__device__ __forceinline__ void fp_store(const FpDom& D, FpTile T, const FpCells& c, u16* f0, float* core) {
+// features (the formulas of feat_pixel, as in cl_fused) into f0
+__device__ __forceinline__ void fp_store(const FpDom& D, FpTile T, const FpCells& c, u16* f0) {
 #pragma unroll
   for (int j = 0; j < FP_PER; j++) {
     const int e = threadIdx.x + 512 * j;
@@ -993,25 +991,17 @@ __device__ __forceinline__ void fp_store(const FpDom& D, FpTile T, const FpCells
       pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
     }
     *(uint4*)(f0 + e * 8) = pk;
-    if (lx >= 0 && lx < FT && ly >= 0 && ly < FT) {
-      float* q = core + (ly * FT + lx) * 3;
-      q[0] = c.qx[j];
-      q[1] = c.qy[j];
-      q[2] = c.h[j];
-    }
   }
 }
 
 extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_eu(2, 2))) void cl_fused_p(
-    const ClDom* __restrict__ dom, int par, const u16* wf, const u16* wl1, const float* w, double* acc,
-    float* o_out, int tx, int ty, int ntiles, int per_wg) {
+    const ClDom* __restrict__ dom, int par, const u16* wf, const u16* wl1, const float* w, float* o_out, int tx,
+    int ty, int ntiles, int per_wg) {
   __shared__ __attribute__((aligned(16))) u16 sw[FP_WN];
   __shared__ float sb[CL_LAYERS * CL_C];
   __shared__ __attribute__((aligned(16))) u16 f0[FR0 * FR0 * 8];
   __shared__ __attribute__((aligned(16))) u16 b1[(FR1 * FR1 + 1) * FPS];  // + a spare pixel slot
   __shared__ __attribute__((aligned(16))) u16 b2[(FR2 * FR2 + 1) * FPS];
-  __shared__ float core[2][FT * FT * 3];
-  __shared__ double red[FNW * 2];
   const int t0 = uni(blockIdx.x * per_wg);
   const int end = uni(t0 + per_wg < ntiles ? t0 + per_wg : ntiles);
   const int cur = par ^ 1;
@@ -1032,10 +1022,8 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
   int tn = fp_next(dom, t + 1, end, tx, ty, par);
   FpTile N = fp_tile(tn < end ? tn : t, tx, ty);
   FpDom DN = fp_dom(dom, N.d, cur);
-  fp_store(DT, T, pf, f0, core[0]);
+  fp_store(DT, T, pf, f0);
   __syncthreads();
-  double v = 0.0, fl = 0.0;
-  int it = 0;
   for (;;) {
     // tile t runs the network; the inputs of tile tn are loaded meanwhile, and the
     // domain descriptor of the tile after it is read (scalar loads, used next iteration)
@@ -1047,63 +1035,27 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
     const FpDom DNN = fp_dom(dom, NN.d, cur);
     __syncthreads();
     fp_layer<FR1, FR2, 2, false>(b1, b2, sw + FP_W1, sb + CL_C, T.ox, T.oy, W, H, nullptr);
-    float* cnext = core[(it + 1) & 1];
-    if (tn < end) fp_store(DN, N, pf, f0, cnext);
+    if (tn < end) fp_store(DN, N, pf, f0);
     __syncthreads();
     fp_layer<FR2, FR3, 2, false>(b2, b1, sw + FP_W2, sb + 2 * CL_C, T.ox, T.oy, W, H, nullptr);
     __syncthreads();
     float* o2 = (float*)b2;
     fp_layer<FR3, FT, 1, false>(b1, nullptr, sw + FP_W3, sb + 3 * CL_C, T.ox, T.oy, W, H, o2);
     __syncthreads();
-    const float* cc = core[it & 1];
     const int e = threadIdx.x;
     const int x = T.ox + e % FT, y = T.oy + e / FT;
     if (e < FT * FT && x < W && y < H) {
       float* od = o_out + ((long)T.d * W * H + (long)y * W + x) * 2;
       od[0] = o2[e * 2];
       od[1] = o2[e * 2 + 1];
-      if (acc) {
-        const float qx = cc[e * 3], qy = cc[e * 3 + 1], hc = cc[e * 3 + 2];
-        const bool rows = y >= 2 && y <= H - 3, cols = x >= 2 && x <= W - 3;
-        if (rows && cols) v += (double)hc;
-        if (rows && x == W - 3) fl += (double)qx;
-        if (rows && x == 1) fl -= (double)qx;
-        if (cols && y == H - 3) fl += (double)qy;
-        if (cols && y == 1) fl -= (double)qy;
-      }
     }
-    const bool last = tn >= end;
-    if (acc && (last || N.d != T.d)) {  // flush the mass terms of domain T.d
-#pragma unroll
-      for (int m = 32; m >= 1; m >>= 1) {
-        v += __shfl_xor(v, m);
-        fl += __shfl_xor(fl, m);
-      }
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      if (lane == 0) {
-        red[2 * wave] = v;
-        red[2 * wave + 1] = fl;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double sv = 0.0, sf = 0.0;
-        for (int k = 0; k < FNW; k++) {
-          sv += red[2 * k];
-          sf += red[2 * k + 1];
-        }
-        atomicAdd(&acc[2 * T.d], sv);
-        atomicAdd(&acc[2 * T.d + 1], sf);
-      }
-      v = fl = 0.0;
-    }
-    if (last) break;
+    if (tn >= end) break;
     t = tn;
     T = N;
     DT = DN;
     tn = tnn;
     N = NN;
     DN = DNN;
-    it++;
   }
 }
 
@@ -1119,9 +1071,8 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
 // step. 16 wavefronts: wavefront 4 L + j runs layer L on pixel tile j (16 columns), and
 // the four wavefronts on a SIMD are one of each layer. Each keeps the A fragments of its layer in registers for
 // the whole march, and each B fragment it reads feeds both of its rows. Every output sums
-// the same fragments in the same order as cl_fused: the same bits. The lanes that load
-// the features also add the mass terms of their rows; cl_correct(_ens) applies the
-// outputs.
+// the same fragments in the same order as cl_fused: the same bits. cl_correct(_ens)
+// applies the outputs.
 #define RS 58                  // output columns per strip
 #define RPX 65                 // pixel slots per ring row: 4 tiles of 16 + a spare slot
 #define RFW (RS + 8)           // feature columns (4 on each side)
@@ -1201,19 +1152,18 @@ __device__ __forceinline__ void rm_load_a(const u16* wp, int NC, int MT, int lan
 
 #define RNW 16                 // wavefronts per work-group of cl_fused_r: 4 per layer
 
-// features of one row for one loader lane, and the row's mass terms
+// features of one row for one loader lane
 struct RmRow {
   float b, h, qx, qy, bs, k;
 };
 
 // One wavefront of layer L (0..3) on pixel tile j over the march steps: reads its input
 // ring `in`, writes its output ring `out` (layer 3: the outputs o; it also loads the
-// features of the next rows, writes them to `out`, the feature ring, and adds their mass
-// terms). Layer L computes rows ya - M + 2k and the next at step k (M = 8, 14, 18, 24).
+// features of the next rows and writes them to `out`, the feature ring). Layer L computes rows ya - M + 2k and the next at step k (M = 8, 14, 18, 24).
 template <int L>
 __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u16* in, u16* out, int j, int lane,
-                                         int x0, int ya, int nrows, int W, int H, float* od, double& v, double& fl,
-                                         bool mass, const ClDom& D, int cur) {
+                                         int x0, int ya, int nrows, int W, int H, float* od, const ClDom& D,
+                                         int cur) {
   constexpr int NC = L == 0 ? 3 : CL_TAPS, MT = L == 3 ? 1 : 2;
   constexpr int RW = RS + 6 - 2 * L;       // region width
   constexpr int M = L == 0 ? 8 : L == 1 ? 14 : L == 2 ? 18 : 24;  // pair k: rows ya - M + 2k, +1
@@ -1234,7 +1184,6 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
   const bool loader = L == 3 && fc >= 0 && fc < RFW;
   const int fx = x0 + fc - 4;
   const bool fxin = fx >= 0 && fx < W;
-  const bool fmass = mass && loader && fc >= 4 && fc < RS + 4 && fx < W;  // a column of the strip
   RmRow nr[2][2];  // the features of pairs k and k + 1 (loads issued two pairs ahead)
   const auto load_row = [&](RmRow& r, int yy) {
     r = RmRow{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -1324,14 +1273,6 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
           pk.z = (uint32_t)f2bf(f4v) | ((uint32_t)f2bf(f5v) << 16);
         }
         *(uint4*)(out + (rm_slot(2 * PH - 4 + r) * RFW + fc) * 8) = pk;  // the feature ring
-        if (fmass && yr >= ya && yr < ya + nrows) {  // mass terms of an owned cell
-          const bool rows = yr >= 2 && yr <= H - 3, cols = fx >= 2 && fx <= W - 3;
-          if (rows && cols) v += (double)w.h;
-          if (rows && fx == W - 3) fl += (double)w.qx;
-          if (rows && fx == 1) fl -= (double)w.qx;
-          if (cols && yr == H - 3) fl += (double)w.qy;
-          if (cols && yr == 1) fl -= (double)w.qy;
-        }
       }
     }
     if (2 * k + 4 < nfeat) {  // pair k + 2: HBM latency hides behind two march steps
@@ -1377,13 +1318,11 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
 
 extern "C" __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __restrict__ dom, int par,
                                                              const u16* wf, const u16* wl1, const float* w,
-                                                             double* acc_out, float* o_out, int nstrip, int nspan,
-                                                             int span) {
+                                                             float* o_out, int nstrip, int nspan, int span) {
   __shared__ __attribute__((aligned(16))) u16 fring[RROWS * RFW * 8];
   __shared__ __attribute__((aligned(16))) u16 ring1[RROWS * RPX * FPS];
   __shared__ __attribute__((aligned(16))) u16 ring2[RROWS * RPX * FPS];
   __shared__ __attribute__((aligned(16))) u16 ring3[RROWS * RPX * FPS];
-  __shared__ double red[RNW * 2];
   const int item = blockIdx.x;
   const int strip = item % nstrip, rest = item / nstrip;
   const int p = rest % nspan, d = rest / nspan;
@@ -1397,41 +1336,17 @@ extern "C" __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int layer = wave >> 2, j = wave & 3;
   float* od = o_out + (long)d * W * H * 2;
-  const bool mass = acc_out != nullptr;
-  double v = 0.0, fl = 0.0;
   const float* bias = w + CL_TAPS * CL_C * CL_C;  // + layer * CL_LAYER_PARAMS
   if (layer == 0)
-    rm_march<0>(wl1, bias, fring, ring1, j, lane, x0, ya, nrows, W, H, od, v, fl, mass, D, cur);
+    rm_march<0>(wl1, bias, fring, ring1, j, lane, x0, ya, nrows, W, H, od, D, cur);
   else if (layer == 1)
-    rm_march<1>(wf + CL_PACKED, bias + CL_LAYER_PARAMS, ring1, ring2, j, lane, x0, ya, nrows, W, H, od, v, fl,
-                mass, D, cur);
+    rm_march<1>(wf + CL_PACKED, bias + CL_LAYER_PARAMS, ring1, ring2, j, lane, x0, ya, nrows, W, H, od, D, cur);
   else if (layer == 2)
-    rm_march<2>(wf + 2 * CL_PACKED, bias + 2 * CL_LAYER_PARAMS, ring2, ring3, j, lane, x0, ya, nrows, W, H, od,
-                v, fl, mass, D, cur);
+    rm_march<2>(wf + 2 * CL_PACKED, bias + 2 * CL_LAYER_PARAMS, ring2, ring3, j, lane, x0, ya, nrows, W, H, od, D,
+                cur);
   else
-    rm_march<3>(wf + 3 * CL_PACKED, bias + 3 * CL_LAYER_PARAMS, ring3, fring, j, lane, x0, ya, nrows, W, H, od,
-                v, fl, mass, D, cur);
-  if (mass) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-      v += __shfl_xor(v, m);
-      fl += __shfl_xor(fl, m);
-    }
-    if (lane == 0) {
-      red[2 * wave] = v;
-      red[2 * wave + 1] = fl;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double sv = 0.0, sf = 0.0;
-      for (int k = 0; k < RNW; k++) {
-        sv += red[2 * k];
-        sf += red[2 * k + 1];
-      }
-      atomicAdd(&acc_out[2 * d], sv);
-      atomicAdd(&acc_out[2 * d + 1], sf);
-    }
-  }
+    rm_march<3>(wf + 3 * CL_PACKED, bias + 3 * CL_LAYER_PARAMS, ring3, fring, j, lane, x0, ya, nrows, W, H, od, D,
+                cur);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1544,25 +1459,25 @@ CL_WRAP(cl_adam, (float* w, float* m, float* v, const float* g, int nlayers, flo
                   float eps, float c1, float c2, hipStream_t st),
         P(w, m, v, g, nlayers, lr, b1, b2, eps, c1, c2), dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))
 CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms, unsigned* part,
-                     hipStream_t st),
-        P(g, par, o, bound, sat, ms, part), dim3((g.W + 255) / 256, g.H), dim3(256))
-CL_WRAP(cl_mass_check, (fip_grid g, int par, double* acc, ClMass* ms, double limit, const unsigned* part,
+                     double* mpart, hipStream_t st),
+        P(g, par, o, bound, sat, ms, part, mpart), dim3((g.W + 255) / 256, g.H), dim3(256))
+CL_WRAP(cl_mass_check, (fip_grid g, int par, const double* mpart, ClMass* ms, double limit, const unsigned* part,
                         int nparts, hipStream_t st),
-        P(g, par, acc, ms, limit, part, nparts), dim3(1), dim3(64))
+        P(g, par, mpart, ms, limit, part, nparts), dim3(1), dim3(64))
 CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, int par, const float* o, float bound, float sat, ClMass* ms,
-                         unsigned* part, int W, int H, int n, hipStream_t st),
-        P(scn, par, o, bound, sat, ms, part), dim3((W + 255) / 256, H, n), dim3(256))
-CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, double* acc, ClMass* ms, double limit,
-                            const unsigned* part, int nparts, hipStream_t st),
-        P(scn, n, par, acc, ms, limit, part, nparts), dim3(n), dim3(64))
+                         unsigned* part, double* mpart, int W, int H, int n, hipStream_t st),
+        P(scn, par, o, bound, sat, ms, part, mpart), dim3((W + 255) / 256, H, n), dim3(256))
+CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, const double* mpart, ClMass* ms,
+                            double limit, const unsigned* part, int nparts, hipStream_t st),
+        P(scn, n, par, mpart, ms, limit, part, nparts), dim3(n), dim3(64))
 CL_WRAP(cl_pack_l1, (const float* w, u16* wl1, hipStream_t st), P(w, wl1), dim3((CL_TAPS * 8 * CL_C + 255) / 256),
         dim3(256))
 
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                           const float* w, double* acc, float* o, hipStream_t st, int form) {
+                           const float* w, float* o, hipStream_t st, int form) {
   const int tx = (W + FT - 1) / FT, ty = (H + FT - 1) / FT;
   if (form == 0) {
-    hipLaunchKernelGGL(cl_fused, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, acc, o);
+    hipLaunchKernelGGL(cl_fused, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, o);
     return hipGetLastError();
   }
   static int cus[64] = {};
@@ -1591,15 +1506,14 @@ hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, co
       if (sp == H) break;
     }
     const int nspan = (H + best - 1) / best;
-    hipLaunchKernelGGL(cl_fused_r, dim3(ndom * nstrip * nspan), dim3(64 * RNW), 0, st, dom, par, wf, wl1, w, acc,
-                       o, nstrip, nspan, best);
+    hipLaunchKernelGGL(cl_fused_r, dim3(ndom * nstrip * nspan), dim3(64 * RNW), 0, st, dom, par, wf, wl1, w, o,
+                       nstrip, nspan, best);
     return hipGetLastError();
   }
   // persistent: one work-group per CU, contiguous runs of tiles
   const int ntiles = tx * ty * ndom;
   const int per = (ntiles + cus[dev] - 1) / cus[dev];
   const int groups = (ntiles + per - 1) / per;
-  hipLaunchKernelGGL(cl_fused_p, dim3(groups), dim3(512), 0, st, dom, par, wf, wl1, w, acc, o, tx, ty, ntiles,
-                     per);
+  hipLaunchKernelGGL(cl_fused_p, dim3(groups), dim3(512), 0, st, dom, par, wf, wl1, w, o, tx, ty, ntiles, per);
   return hipGetLastError();
 }

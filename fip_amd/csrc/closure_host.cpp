@@ -114,7 +114,7 @@ struct Attachment {
   float* zref = nullptr;    // device, one per scenario (ensemble)
   float zref_h = 0.f;
   ClMass* ms = nullptr;     // one per scenario
-  double* acc = nullptr;    // 2 per scenario
+  double* mpart = nullptr;  // mass-balance terms per correction work-group, [scenario][nparts][2]
   ClDom* dom = nullptr;     // one per scenario
   float* o = nullptr;       // network outputs, [scenario][H][W][2]
   unsigned* part = nullptr; // saturated faces per correction work-group, [scenario][nparts]
@@ -127,7 +127,7 @@ struct Attachment {
     if (bstd) hipFree(bstd);
     if (zref) hipFree(zref);
     if (ms) hipFree(ms);
-    if (acc) hipFree(acc);
+    if (mpart) hipFree(mpart);
     if (dom) hipFree(dom);
   }
 };
@@ -549,11 +549,10 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   if (!rc) rc = dalloc(&a->o, spx * 2);
   if (!rc) rc = dalloc(&a->part, (size_t)a->nparts);
   if (!rc) rc = dalloc(&a->ms, 1);
-  if (!rc) rc = dalloc(&a->acc, 2);
+  if (!rc) rc = dalloc(&a->mpart, (size_t)2 * a->nparts);
   if (rc) return rc;
   CHIP(hipMemcpy(a->bstd, bed_std, spx * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemset(a->ms, 0, sizeof(ClMass)));
-  CHIP(hipMemset(a->acc, 0, 2 * sizeof(double)));
   ClDom hd{g, a->bstd, zref, 0};
   if ((rc = dalloc(&a->dom, 1))) return rc;
   CHIP(hipMemcpy(a->dom, &hd, sizeof hd, hipMemcpyHostToDevice));
@@ -561,9 +560,9 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   cl->att.push_back(std::move(a));
   fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
     const fip_grid& g = fip_internal::grid(ap->sim);
-    CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, ap->acc, ap->o, st, cl->fused_form));
-    CHIP(cl_correct_launch(g, par, ap->o, kBound, ap->sat, ap->ms, ap->part, st));
-    CHIP(cl_mass_check_launch(g, par, ap->acc, ap->ms, ap->limit, ap->part, ap->nparts, st));
+    CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form));
+    CHIP(cl_correct_launch(g, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, st));
+    CHIP(cl_mass_check_launch(g, par, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
   });
   return FIP_OK;
@@ -595,12 +594,11 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   if (!rc) rc = dalloc(&a->part, (size_t)a->nparts * n);
   if (!rc) rc = dalloc(&a->zref, (size_t)n);
   if (!rc) rc = dalloc(&a->ms, (size_t)n);
-  if (!rc) rc = dalloc(&a->acc, (size_t)2 * n);
+  if (!rc) rc = dalloc(&a->mpart, (size_t)2 * a->nparts * n);
   if (rc) return rc;
   CHIP(hipMemcpy(a->bstd, bed_std, spx * n * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemcpy(a->zref, zref, n * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemset(a->ms, 0, n * sizeof(ClMass)));
-  CHIP(hipMemset(a->acc, 0, 2 * n * sizeof(double)));
   std::vector<ClDom> hd(n);
   for (int s2 = 0; s2 < n; s2++)
     hd[s2] = ClDom{fip_internal::grid(fip_internal::ens_member(ens, s2)), a->bstd + (size_t)s2 * spx, zref[s2], 0};
@@ -611,9 +609,9 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   fip_internal::ens_set_post_hook(ens, [cl, ap, n](hipStream_t st, int par) -> int {
     const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ap->ens, 0));
     const fip_ens_scn* scn = fip_internal::ens_scn(ap->ens);
-    CHIP(cl_fused_launch(ap->dom, n, g0.W, g0.H, par, cl->wf, cl->wl1, cl->w, ap->acc, ap->o, st, cl->fused_form));
-    CHIP(cl_correct_ens_launch(scn, par, ap->o, kBound, ap->sat, ap->ms, ap->part, g0.W, g0.H, n, st));
-    CHIP(cl_mass_check_ens_launch(scn, n, par, ap->acc, ap->ms, ap->limit, ap->part, ap->nparts, st));
+    CHIP(cl_fused_launch(ap->dom, n, g0.W, g0.H, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form));
+    CHIP(cl_correct_ens_launch(scn, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, g0.W, g0.H, n, st));
+    CHIP(cl_mass_check_ens_launch(scn, n, par, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
   });
   return FIP_OK;
@@ -738,7 +736,7 @@ int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed
   CHIP(hipMemcpy(dom, &hd, sizeof hd, hipMemcpyHostToDevice));
   // par such that par ^ 1 is the current side and the halt flag read is that of no step
   const int par = fip_internal::cur_side(coarse) ^ 1;
-  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, nullptr, o, nullptr, cl->fused_form);
+  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, o, nullptr, cl->fused_form);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(out, o, spx * 2 * sizeof(float), hipMemcpyDeviceToHost);
   hipFree(bstd); hipFree(o); hipFree(dom);
