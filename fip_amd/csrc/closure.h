@@ -79,9 +79,11 @@ hipError_t cl_loss_sum_launch(const double* part, int nb, float* out, hipStream_
 hipError_t cl_adam_launch(float* w, float* m, float* v, const float* g, int nlayers, float lr, float b1,
                           float b2, float eps, float c1, float c2, hipStream_t st);
 // Correction of the current face discharges by o; saturated faces counted and the
-// mass-balance terms of the uncorrected state summed per work-group of 256 columns of a
-// row into part and mpart (2 doubles per slot; (W + 255) / 256 * H slots per domain),
-// which the mass check adds in a fixed order.
+// mass-balance terms of the uncorrected state summed per work-group of 256 columns of
+// CL_CORRECT_ROWS rows into part and mpart (2 doubles per slot; cl_correct_parts(W, H)
+// slots per domain), which the mass check adds in a fixed order.
+#define CL_CORRECT_ROWS 8
+inline int cl_correct_parts(int W, int H) { return (W + 255) / 256 * ((H + CL_CORRECT_ROWS - 1) / CL_CORRECT_ROWS); }
 hipError_t cl_correct_launch(fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms,
                              unsigned* part, double* mpart, hipStream_t st);
 hipError_t cl_mass_check_launch(fip_grid g, int par, const double* mpart, ClMass* ms, double limit,

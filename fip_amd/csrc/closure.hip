@@ -547,17 +547,31 @@ __device__ __forceinline__ unsigned wave_sum_parts(const unsigned* part, const d
 // q += tanh(o) * bound * |q|, unless |o| exceeds `sat` (the network saturates: that face
 // keeps the plain step) or the domain's fallback flag is set. A corrected face moves
 // water between two cells of the domain, so the volume is conserved by construction.
-// One work-group per 256 columns of a row; its count of saturated faces goes to
-// part[blockIdx.y * gridDim.x + blockIdx.x].
+// One work-group per 256 columns of CL_CORRECT_ROWS rows; its count of saturated faces and
+// its mass terms go to slot blockIdx.y * gridDim.x + blockIdx.x of part and mpart.
+__device__ __forceinline__ unsigned correct_rows(const fip_grid& g, int cur, const float* o, float bound, float sat,
+                                                 bool off, double& v, double& f) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned nsat = 0;
+  v = 0.0;
+  f = 0.0;
+  if (x < g.W)
+    for (int r = 0; r < CL_CORRECT_ROWS; r++) {
+      const int y = blockIdx.y * CL_CORRECT_ROWS + r;
+      if (y >= g.H) break;
+      double cv, cf;
+      nsat += correct_cell(g, cur, o, x, y, bound, sat, off, cv, cf);
+      v += cv;
+      f += cf;
+    }
+  return nsat;
+}
+
 extern "C" __global__ __launch_bounds__(256) void cl_correct(fip_grid g, int par, const float* o, float bound,
                                                             float sat, ClMass* ms, unsigned* part, double* mpart) {
   if (g.ctl->halt_req[par ^ 1]) return;
-  const int cur = par ^ 1;
-  const bool off = ms->fallback != 0;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  unsigned nsat = 0;
-  double v = 0.0, f = 0.0;
-  if (x < g.W && y < g.H) nsat = correct_cell(g, cur, o, x, y, bound, sat, off, v, f);
+  double v, f;
+  const unsigned nsat = correct_rows(g, par ^ 1, o, bound, sat, ms->fallback != 0, v, f);
   const long slot = (long)blockIdx.y * gridDim.x + blockIdx.x;
   block_count(part + slot, mpart + 2 * slot, nsat, v, f);
 }
@@ -608,13 +622,9 @@ extern "C" __global__ __launch_bounds__(256) void cl_correct_ens(const fip_ens_s
   const int s = blockIdx.z;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
-  const int cur = par ^ 1;
-  const bool off = ms[s].fallback != 0;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  unsigned nsat = 0;
-  double v = 0.0, f = 0.0;
-  if (x < g.W && y < g.H)
-    nsat = correct_cell(g, cur, o + (long)s * g.W * g.H * 2, x, y, bound, sat, off, v, f);
+  double v, f;
+  const unsigned nsat =
+      correct_rows(g, par ^ 1, o + (long)s * g.W * g.H * 2, bound, sat, ms[s].fallback != 0, v, f);
   const long nparts = (long)gridDim.x * gridDim.y;
   const long slot = s * nparts + (long)blockIdx.y * gridDim.x + blockIdx.x;
   block_count(part + slot, mpart + 2 * slot, nsat, v, f);
@@ -1460,13 +1470,15 @@ CL_WRAP(cl_adam, (float* w, float* m, float* v, const float* g, int nlayers, flo
         P(w, m, v, g, nlayers, lr, b1, b2, eps, c1, c2), dim3((nlayers * CL_LAYER_PARAMS + 255) / 256), dim3(256))
 CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms, unsigned* part,
                      double* mpart, hipStream_t st),
-        P(g, par, o, bound, sat, ms, part, mpart), dim3((g.W + 255) / 256, g.H), dim3(256))
+        P(g, par, o, bound, sat, ms, part, mpart),
+        dim3((g.W + 255) / 256, (g.H + CL_CORRECT_ROWS - 1) / CL_CORRECT_ROWS), dim3(256))
 CL_WRAP(cl_mass_check, (fip_grid g, int par, const double* mpart, ClMass* ms, double limit, const unsigned* part,
                         int nparts, hipStream_t st),
         P(g, par, mpart, ms, limit, part, nparts), dim3(1), dim3(64))
 CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, int par, const float* o, float bound, float sat, ClMass* ms,
                          unsigned* part, double* mpart, int W, int H, int n, hipStream_t st),
-        P(scn, par, o, bound, sat, ms, part, mpart), dim3((W + 255) / 256, H, n), dim3(256))
+        P(scn, par, o, bound, sat, ms, part, mpart),
+        dim3((W + 255) / 256, (H + CL_CORRECT_ROWS - 1) / CL_CORRECT_ROWS, n), dim3(256))
 CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, const double* mpart, ClMass* ms,
                             double limit, const unsigned* part, int nparts, hipStream_t st),
         P(scn, n, par, mpart, ms, limit, part, nparts), dim3(n), dim3(64))

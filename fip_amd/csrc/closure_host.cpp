@@ -544,7 +544,7 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   a->limit = mass_limit;
   const fip_grid& g = fip_internal::grid(coarse);
   const size_t spx = (size_t)g.W * g.H;
-  a->nparts = (g.W + 255) / 256 * g.H;
+  a->nparts = cl_correct_parts(g.W, g.H);
   int rc = dalloc(&a->bstd, spx);
   if (!rc) rc = dalloc(&a->o, spx * 2);
   if (!rc) rc = dalloc(&a->part, (size_t)a->nparts);
@@ -588,7 +588,7 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   a->sat = sat;
   a->limit = mass_limit;
   const size_t spx = (size_t)g0.W * g0.H;
-  a->nparts = (g0.W + 255) / 256 * g0.H;
+  a->nparts = cl_correct_parts(g0.W, g0.H);
   int rc = dalloc(&a->bstd, spx * n);
   if (!rc) rc = dalloc(&a->o, spx * 2 * n);
   if (!rc) rc = dalloc(&a->part, (size_t)a->nparts * n);
