@@ -269,11 +269,13 @@ def test_fused_kernel_forms_agree(shape, form):
 
 def test_closure_ensemble_deterministic():
     """Two runs of a closure-corrected ensemble give the same bits: the network reads the
-    uncorrected state of every tile's halo, and the correction is a separate launch."""
+    uncorrected state of every tile's halo, and the correction is a separate launch. The
+    mass check's figures are the same too (its sums are added in a fixed order), and the
+    relative mass error stays at rounding level."""
     from fip_amd.sim import Closure, Ensemble
     rng = np.random.default_rng(12)
     p = rand_params(rng, 0.3)
-    runs = []
+    runs, status = [], []
     for _ in range(2):
         cases, stds, zrefs = [], [], []
         for sc in (21, 22, 23):
@@ -288,10 +290,14 @@ def test_closure_ensemble_deterministic():
         e.set_run([1e9] * 3)
         e.simulate(40)
         runs.append([e.field(k, f) for k in range(3) for f in ("h", "qx", "qy")])
+        status.append([cl.status(ens=e, s=k) for k in range(3)])
         e.close()
         cl.close()
     for a, b in zip(*runs):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert status[0] == status[1], status
+    for st in status[0]:
+        assert st["fallback"] == 0 and 0.0 <= st["max_rel_mass_error"] < 1e-6, st
 
 
 def test_adam_step_matches_oracle():
