@@ -134,10 +134,11 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
     fines = [fine_case(s, W=W, H=W, duration=duration) for s in scenarios]
     pairs = [coarse_of(f) for f in fines]
     coarses = [p[0] for p in pairs]
-    bstd = np.stack([p[1] for p in pairs])
+    bstd = np.concatenate([p[1].ravel() for p in pairs])  # each scenario's in turn (sizes may differ)
     zref = np.array([p[2] for p in pairs], np.float32)
-    out = {"scenarios": len(scenarios), "duration_s": duration, "fine": f"{fines[0].W}x{fines[0].H} dx=1",
-           "coarse": f"{coarses[0].W}x{coarses[0].H} dx=4"}
+    out = {"scenarios": len(scenarios), "duration_s": duration,
+           "fine": ", ".join(sorted({f"{f.W}x{f.H}" for f in fines})) + " dx=1",
+           "coarse": ", ".join(sorted({f"{c.W}x{c.H}" for c in coarses})) + " dx=4"}
 
     def run(cases, closure):
         # each ensemble runs with the launch shape the ensemble autotuner picks for it

@@ -1,5 +1,6 @@
 // Shared by the closure kernels (closure.hip) and their host code (closure_host.cpp).
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -44,15 +45,23 @@ struct ClMass {
 struct ClDom {
   fip_grid g;
   const float* bstd;   // W x H dense
+  long long o_off;     // pixel offset of the domain's output image ([H][W][2]) in o
   float zref;
   int pad;
 };
 #define CL_PACKED_L1 (2 * 3 * 64 * 8)   // bf16 elements of the 8-channel packing of layer 0
 
-// Network outputs o ([ndom][H][W][2]) of every cell of ndom domains of one size. form 2:
-// row march, 1: persistent tiles, 0: one work-group per tile.
+// Network outputs o of every cell of ndom domains (domain d's [H][W][2] image at pixel
+// offset dom[d].o_off; the domains may differ in size, W x H bounds them all). form 2:
+// row march over the work items of cl_fused_plan (items, nitems), 1: persistent tiles,
+// 0: one work-group per tile.
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                           const float* w, float* o, hipStream_t st, int form = 1);
+                           const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems);
+// The row march's work items (domain, first column, first row, end row) for domains of
+// sizes W[d] x H[d] on a device of `cus` compute units: strips of the kernel's width and
+// spans of one length, chosen so that the items (one work-group per compute unit at a
+// time) finish in the fewest march steps.
+std::vector<int4> cl_fused_plan(const int* W, const int* H, int ndom, int cus);
 hipError_t cl_pack_l1_launch(const float* w, u16* wl1, hipStream_t st);
 hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
                           u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,
@@ -88,7 +97,10 @@ hipError_t cl_correct_launch(fip_grid g, int par, const float* o, float bound, f
                              unsigned* part, double* mpart, hipStream_t st);
 hipError_t cl_mass_check_launch(fip_grid g, int par, const double* mpart, ClMass* ms, double limit,
                                 const unsigned* part, int nparts, hipStream_t st);
-hipError_t cl_correct_ens_launch(const fip_ens_scn* scn, int par, const float* o, float bound, float sat,
-                                 ClMass* ms, unsigned* part, double* mpart, int W, int H, int n, hipStream_t st);
+// (ensembles: dom[s].o_off places scenario s's outputs in o; W x H bounds every scenario,
+// and a scenario's work-groups past its own size write empty slots)
+hipError_t cl_correct_ens_launch(const fip_ens_scn* scn, const ClDom* dom, int par, const float* o, float bound,
+                                 float sat, ClMass* ms, unsigned* part, double* mpart, int W, int H, int n,
+                                 hipStream_t st);
 hipError_t cl_mass_check_ens_launch(const fip_ens_scn* scn, int n, int par, const double* mpart, ClMass* ms,
                                     double limit, const unsigned* part, int nparts, hipStream_t st);

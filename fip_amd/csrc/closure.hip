@@ -616,15 +616,15 @@ extern "C" __global__ void cl_mass_check(fip_grid g, int par, const double* mpar
 // Ensemble forms of cl_correct / cl_mass_check: blockIdx.z (cl_mass_check_ens:
 // blockIdx.x) = scenario; o holds one W x H image per scenario, part nparts slots per
 // scenario.
-extern "C" __global__ __launch_bounds__(256) void cl_correct_ens(const fip_ens_scn* scn, int par, const float* o,
-                                                                float bound, float sat, ClMass* ms,
+extern "C" __global__ __launch_bounds__(256) void cl_correct_ens(const fip_ens_scn* scn, const ClDom* dom, int par,
+                                                                const float* o, float bound, float sat, ClMass* ms,
                                                                 unsigned* part, double* mpart) {
   const int s = blockIdx.z;
   const fip_grid& g = scn[s].g;
   if (g.ctl->halt_req[par ^ 1]) return;
-  double v, f;
+  double v, f;  // work-groups past a smaller scenario's size write empty slots
   const unsigned nsat =
-      correct_rows(g, par ^ 1, o + (long)s * g.W * g.H * 2, bound, sat, ms[s].fallback != 0, v, f);
+      correct_rows(g, par ^ 1, o + dom[s].o_off * 2, bound, sat, ms[s].fallback != 0, v, f);
   const long nparts = (long)gridDim.x * gridDim.y;
   const long slot = s * nparts + (long)blockIdx.y * gridDim.x + blockIdx.x;
   block_count(part + slot, mpart + 2 * slot, nsat, v, f);
@@ -796,7 +796,7 @@ extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, 
                                  ox, oy, W, H, o2);
   __syncthreads();
   // the outputs of the tile's cells into o (one [H][W][2] image per domain)
-  float* od = o_out + (long)d * W * H * 2;
+  float* od = o_out + dom[d].o_off * 2;
   for (int e = threadIdx.x; e < FT * FT; e += blockDim.x) {
     const int x = ox + e % FT, y = oy + e / FT;
     if (x >= W || y >= H) continue;
@@ -1055,7 +1055,7 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
     const int e = threadIdx.x;
     const int x = T.ox + e % FT, y = T.oy + e / FT;
     if (e < FT * FT && x < W && y < H) {
-      float* od = o_out + ((long)T.d * W * H + (long)y * W + x) * 2;
+      float* od = o_out + (dom[T.d].o_off + (long)y * W + x) * 2;
       od[0] = o2[e * 2];
       od[1] = o2[e * 2 + 1];
     }
@@ -1328,24 +1328,22 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
 
 extern "C" __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __restrict__ dom, int par,
                                                              const u16* wf, const u16* wl1, const float* w,
-                                                             float* o_out, int nstrip, int nspan, int span) {
+                                                             float* o_out, const int4* __restrict__ items) {
   __shared__ __attribute__((aligned(16))) u16 fring[RROWS * RFW * 8];
   __shared__ __attribute__((aligned(16))) u16 ring1[RROWS * RPX * FPS];
   __shared__ __attribute__((aligned(16))) u16 ring2[RROWS * RPX * FPS];
   __shared__ __attribute__((aligned(16))) u16 ring3[RROWS * RPX * FPS];
-  const int item = blockIdx.x;
-  const int strip = item % nstrip, rest = item / nstrip;
-  const int p = rest % nspan, d = rest / nspan;
+  const int4 it = items[blockIdx.x];  // (domain, first column, first row, end row)
+  const int d = uni(it.x);
   const ClDom& D = dom[d];
   if (D.g.ctl->halt_req[par ^ 1]) return;  // the whole work-group
   const int cur = par ^ 1;
   const int W = D.g.W, H = D.g.H;
-  const int x0 = strip * RS, ya = p * span;
-  const int yb = ya + span < H ? ya + span : H;
+  const int x0 = uni(it.y), ya = uni(it.z), yb = uni(it.w);
   const int nrows = yb - ya;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int layer = wave >> 2, j = wave & 3;
-  float* od = o_out + (long)d * W * H * 2;
+  float* od = o_out + D.o_off * 2;
   const float* bias = w + CL_TAPS * CL_C * CL_C;  // + layer * CL_LAYER_PARAMS
   if (layer == 0)
     rm_march<0>(wl1, bias, fring, ring1, j, lane, x0, ya, nrows, W, H, od, D, cur);
@@ -1475,9 +1473,9 @@ CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat
 CL_WRAP(cl_mass_check, (fip_grid g, int par, const double* mpart, ClMass* ms, double limit, const unsigned* part,
                         int nparts, hipStream_t st),
         P(g, par, mpart, ms, limit, part, nparts), dim3(1), dim3(64))
-CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, int par, const float* o, float bound, float sat, ClMass* ms,
-                         unsigned* part, double* mpart, int W, int H, int n, hipStream_t st),
-        P(scn, par, o, bound, sat, ms, part, mpart),
+CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, const ClDom* dom, int par, const float* o, float bound, float sat,
+                         ClMass* ms, unsigned* part, double* mpart, int W, int H, int n, hipStream_t st),
+        P(scn, dom, par, o, bound, sat, ms, part, mpart),
         dim3((W + 255) / 256, (H + CL_CORRECT_ROWS - 1) / CL_CORRECT_ROWS, n), dim3(256))
 CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, const double* mpart, ClMass* ms,
                             double limit, const unsigned* part, int nparts, hipStream_t st),
@@ -1485,11 +1483,42 @@ CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, const double
 CL_WRAP(cl_pack_l1, (const float* w, u16* wl1, hipStream_t st), P(w, wl1), dim3((CL_TAPS * 8 * CL_C + 255) / 256),
         dim3(256))
 
+std::vector<int4> cl_fused_plan(const int* W, const int* H, int ndom, int cus) {
+  // strips of RS columns, spans of rows chosen so that the work-groups (one per CU at a
+  // time) finish in the fewest march steps, (span + 25) / 2 each
+  int hmax = 1;
+  for (int d = 0; d < ndom; d++) hmax = H[d] > hmax ? H[d] : hmax;
+  int best = hmax;
+  long best_cost = -1;
+  for (int span = 16; span < hmax + 16; span += 16) {
+    const int sp = span < hmax ? span : hmax;
+    long items = 0;
+    for (int d = 0; d < ndom; d++) items += (long)((W[d] + RS - 1) / RS) * ((H[d] + sp - 1) / sp);
+    const long rounds = (items + cus - 1) / cus;
+    const long cost = rounds * ((sp + 25) / 2);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = sp;
+    }
+    if (sp == hmax) break;
+  }
+  std::vector<int4> out;
+  for (int d = 0; d < ndom; d++)
+    for (int ya = 0; ya < H[d]; ya += best)
+      for (int x0 = 0; x0 < W[d]; x0 += RS) out.push_back(make_int4(d, x0, ya, ya + best < H[d] ? ya + best : H[d]));
+  return out;
+}
+
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                           const float* w, float* o, hipStream_t st, int form) {
+                           const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems) {
   const int tx = (W + FT - 1) / FT, ty = (H + FT - 1) / FT;
   if (form == 0) {
     hipLaunchKernelGGL(cl_fused, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, o);
+    return hipGetLastError();
+  }
+  if (form == 2) {
+    if (!items || nitems < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cl_fused_r, dim3(nitems), dim3(64 * RNW), 0, st, dom, par, wf, wl1, w, o, items);
     return hipGetLastError();
   }
   static int cus[64] = {};
@@ -1500,27 +1529,6 @@ hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, co
   if (!cus[dev]) {
     e = hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-  }
-  if (form == 2) {
-    // row march: strips of RS columns, spans of rows chosen so that the work-groups (one
-    // per CU at a time) finish in the fewest march steps, (span + 25) / 2 each
-    const int nstrip = (W + RS - 1) / RS;
-    int best = H, best_cost = -1;
-    for (int span = 16; span < H + 16; span += 16) {
-      const int sp = span < H ? span : H;
-      const long items = (long)ndom * nstrip * ((H + sp - 1) / sp);
-      const long rounds = (items + cus[dev] - 1) / cus[dev];
-      const long cost = rounds * ((sp + 25) / 2);
-      if (best_cost < 0 || cost < best_cost) {
-        best_cost = cost;
-        best = sp;
-      }
-      if (sp == H) break;
-    }
-    const int nspan = (H + best - 1) / best;
-    hipLaunchKernelGGL(cl_fused_r, dim3(ndom * nstrip * nspan), dim3(64 * RNW), 0, st, dom, par, wf, wl1, w, o,
-                       nstrip, nspan, best);
-    return hipGetLastError();
   }
   // persistent: one work-group per CU, contiguous runs of tiles
   const int ntiles = tx * ty * ndom;

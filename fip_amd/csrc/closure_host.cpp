@@ -116,7 +116,10 @@ struct Attachment {
   ClMass* ms = nullptr;     // one per scenario
   double* mpart = nullptr;  // mass-balance terms per correction work-group, [scenario][nparts][2]
   ClDom* dom = nullptr;     // one per scenario
-  float* o = nullptr;       // network outputs, [scenario][H][W][2]
+  int4* items = nullptr;    // work items of the row-march network kernel (cl_fused_plan)
+  int nitems = 0;
+  int wmax = 0, hmax = 0;   // the largest scenario size
+  float* o = nullptr;       // network outputs, each scenario's [H][W][2] image in turn
   unsigned* part = nullptr; // saturated faces per correction work-group, [scenario][nparts]
   int nparts = 0;
   float sat = 2.65f;
@@ -129,8 +132,25 @@ struct Attachment {
     if (ms) hipFree(ms);
     if (mpart) hipFree(mpart);
     if (dom) hipFree(dom);
+    if (items) hipFree(items);
   }
 };
+
+int device_cus(int dev) {
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+  return n;
+}
+
+// the row march's work items for domains of the given sizes, uploaded to *items
+int upload_plan(const std::vector<int>& W, const std::vector<int>& H, int dev, int4** items, int* nitems) {
+  const std::vector<int4> it = cl_fused_plan(W.data(), H.data(), (int)W.size(), device_cus(dev));
+  int rc = dalloc(items, it.size());
+  if (rc) return rc;
+  CHIP(hipMemcpy(*items, it.data(), it.size() * sizeof(int4), hipMemcpyHostToDevice));
+  *nitems = (int)it.size();
+  return FIP_OK;
+}
 
 }  // namespace
 
@@ -553,14 +573,16 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   if (rc) return rc;
   CHIP(hipMemcpy(a->bstd, bed_std, spx * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemset(a->ms, 0, sizeof(ClMass)));
-  ClDom hd{g, a->bstd, zref, 0};
+  ClDom hd{g, a->bstd, 0, zref, 0};
   if ((rc = dalloc(&a->dom, 1))) return rc;
   CHIP(hipMemcpy(a->dom, &hd, sizeof hd, hipMemcpyHostToDevice));
+  if ((rc = upload_plan({g.W}, {g.H}, cl->dev, &a->items, &a->nitems))) return rc;
   Attachment* ap = a.get();
   cl->att.push_back(std::move(a));
   fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
     const fip_grid& g = fip_internal::grid(ap->sim);
-    CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form));
+    CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form, ap->items,
+                         ap->nitems));
     CHIP(cl_correct_launch(g, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, st));
     CHIP(cl_mass_check_launch(g, par, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
@@ -573,10 +595,13 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   if (!cl || !ens || !bed_std || !zref) return fail(FIP_E_INVALID, "null argument");
   CHIP(hipSetDevice(cl->dev));
   const int n = fip_internal::ens_count(ens);
-  const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ens, 0));
-  for (int s = 1; s < n; s++) {
+  std::vector<int> Ws(n), Hs(n);
+  std::vector<long long> off(n + 1, 0);
+  for (int s = 0; s < n; s++) {
     const fip_grid& g = fip_internal::grid(fip_internal::ens_member(ens, s));
-    if (g.W != g0.W || g.H != g0.H) return fail(FIP_E_INVALID, "every scenario must have one coarse size");
+    Ws[s] = g.W;
+    Hs[s] = g.H;
+    off[s + 1] = off[s] + (long long)g.W * g.H;
   }
   for (auto it = cl->att.begin(); it != cl->att.end(); ++it)
     if ((*it)->ens == ens) {
@@ -587,30 +612,34 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   a->ens = ens;
   a->sat = sat;
   a->limit = mass_limit;
-  const size_t spx = (size_t)g0.W * g0.H;
-  a->nparts = cl_correct_parts(g0.W, g0.H);
-  int rc = dalloc(&a->bstd, spx * n);
-  if (!rc) rc = dalloc(&a->o, spx * 2 * n);
+  a->wmax = *std::max_element(Ws.begin(), Ws.end());
+  a->hmax = *std::max_element(Hs.begin(), Hs.end());
+  const size_t npx = (size_t)off[n];
+  a->nparts = cl_correct_parts(a->wmax, a->hmax);  // per scenario, at the largest size
+  int rc = dalloc(&a->bstd, npx);
+  if (!rc) rc = dalloc(&a->o, npx * 2);
   if (!rc) rc = dalloc(&a->part, (size_t)a->nparts * n);
   if (!rc) rc = dalloc(&a->zref, (size_t)n);
   if (!rc) rc = dalloc(&a->ms, (size_t)n);
   if (!rc) rc = dalloc(&a->mpart, (size_t)2 * a->nparts * n);
   if (rc) return rc;
-  CHIP(hipMemcpy(a->bstd, bed_std, spx * n * sizeof(float), hipMemcpyHostToDevice));
+  CHIP(hipMemcpy(a->bstd, bed_std, npx * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemcpy(a->zref, zref, n * sizeof(float), hipMemcpyHostToDevice));
   CHIP(hipMemset(a->ms, 0, n * sizeof(ClMass)));
   std::vector<ClDom> hd(n);
   for (int s2 = 0; s2 < n; s2++)
-    hd[s2] = ClDom{fip_internal::grid(fip_internal::ens_member(ens, s2)), a->bstd + (size_t)s2 * spx, zref[s2], 0};
+    hd[s2] = ClDom{fip_internal::grid(fip_internal::ens_member(ens, s2)), a->bstd + off[s2], off[s2], zref[s2], 0};
   if ((rc = dalloc(&a->dom, (size_t)n))) return rc;
   CHIP(hipMemcpy(a->dom, hd.data(), n * sizeof(ClDom), hipMemcpyHostToDevice));
+  if ((rc = upload_plan(Ws, Hs, cl->dev, &a->items, &a->nitems))) return rc;
   Attachment* ap = a.get();
   cl->att.push_back(std::move(a));
   fip_internal::ens_set_post_hook(ens, [cl, ap, n](hipStream_t st, int par) -> int {
-    const fip_grid& g0 = fip_internal::grid(fip_internal::ens_member(ap->ens, 0));
     const fip_ens_scn* scn = fip_internal::ens_scn(ap->ens);
-    CHIP(cl_fused_launch(ap->dom, n, g0.W, g0.H, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form));
-    CHIP(cl_correct_ens_launch(scn, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, g0.W, g0.H, n, st));
+    CHIP(cl_fused_launch(ap->dom, n, ap->wmax, ap->hmax, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form,
+                         ap->items, ap->nitems));
+    CHIP(cl_correct_ens_launch(scn, ap->dom, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, ap->wmax,
+                               ap->hmax, n, st));
     CHIP(cl_mass_check_ens_launch(scn, n, par, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
   });
@@ -732,14 +761,18 @@ int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed
   CHIP(hipMalloc(&o, spx * 2 * sizeof(float)));
   CHIP(hipMalloc(&dom, sizeof(ClDom)));
   CHIP(hipMemcpy(bstd, bed_std, spx * sizeof(float), hipMemcpyHostToDevice));
-  ClDom hd{g, bstd, zref, 0};
+  ClDom hd{g, bstd, 0, zref, 0};
   CHIP(hipMemcpy(dom, &hd, sizeof hd, hipMemcpyHostToDevice));
+  int4* items = nullptr;
+  int nitems = 0;
+  if (int rc = upload_plan({g.W}, {g.H}, cl->dev, &items, &nitems)) return rc;
   // par such that par ^ 1 is the current side and the halt flag read is that of no step
   const int par = fip_internal::cur_side(coarse) ^ 1;
-  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, o, nullptr, cl->fused_form);
+  hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, o, nullptr, cl->fused_form, items,
+                                 nitems);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(out, o, spx * 2 * sizeof(float), hipMemcpyDeviceToHost);
-  hipFree(bstd); hipFree(o); hipFree(dom);
+  hipFree(bstd); hipFree(o); hipFree(dom); hipFree(items);
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("fused outputs: ") + hipGetErrorString(e));
   return FIP_OK;
 }

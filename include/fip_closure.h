@@ -73,8 +73,9 @@ int fip_closure_eval(fip_closure* cl, int first, int n, double* mse_plain, doubl
  * a relative error above mass_limit switches the domain to the plain step for good. */
 int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, float zref,
                        float sat, double mass_limit);
-/* The same for every scenario of an ensemble (all scenarios one coarse size);
- * bed_std holds n x Wc x Hc values, zref n values. */
+/* The same for every scenario of an ensemble; the scenarios may differ in size. bed_std
+ * holds each scenario's W x H values in turn (n x W x H when they have one size), zref n
+ * values. */
 int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_std,
                                 const float* zref, float sat, double mass_limit);
 int fip_closure_detach(fip_sim* coarse);

@@ -300,6 +300,49 @@ def test_closure_ensemble_deterministic():
         assert st["fallback"] == 0 and 0.0 <= st["max_rel_mass_error"] < 1e-6, st
 
 
+def test_closure_ensemble_of_mixed_sizes_equals_single_domains():
+    """An ensemble of coarse scenarios of different sizes with the closure attached (the
+    held-out workload asks for coarse grids of 512^2 to 1024^2 in one batched ensemble):
+    every scenario equals the same scenario run alone with the closure attached, bit for
+    bit, after steps in which the correction acts."""
+    from fip_amd.sim import Closure, Ensemble, Sim
+    rng = np.random.default_rng(31)
+    p = rand_params(rng, 0.3)
+    sizes = [(320, 256), (256, 256), (448, 192)]
+    cases, stds, zrefs = [], [], []
+    for k, (W, H) in enumerate(sizes):
+        _, coarse, bstd, zref = fine_coarse_case(W, H, 1.0, 4, scenario=40 + k, wet=0.5)
+        cases.append(coarse)
+        stds.append(bstd)
+        zrefs.append(zref)
+    cl = Closure(64, 64, 0, seed=2)
+    cl.params = p
+    e = Ensemble(cases, mode="dense", variant="fast", tile_rows=32)
+    cl.attach_ensemble(e, np.concatenate([b.ravel() for b in stds]), np.array(zrefs, np.float32), sat=3.0)
+    e.set_run([1e9] * len(cases))
+    e.simulate(30)
+    ens = [[e.field(k, f) for f in ("h", "qx", "qy")] for k in range(len(cases))]
+    st = [cl.status(ens=e, s=k) for k in range(len(cases))]
+    e.close()
+    for k, case in enumerate(cases):
+        s = Sim(case, mode="dense", variant="fast")
+        cl.attach(s, stds[k], zrefs[k], sat=3.0)
+        s.set_run(1e9)
+        s.simulate(30)
+        one = [s.field(f) for f in ("h", "qx", "qy")]
+        one_st = cl.status(s)
+        # the counts are the same; the mass sums add the same terms in another order (the
+        # ensemble's slots are laid out for its largest scenario)
+        assert (one_st["fallback"], one_st["saturated_faces"]) == (st[k]["fallback"], st[k]["saturated_faces"])
+        assert abs(one_st["max_rel_mass_error"] - st[k]["max_rel_mass_error"]) < 1e-9
+        Closure.detach(s)
+        s.close()
+        for a, b in zip(one, ens[k]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
+    cl.close()
+    assert max(x["max_rel_mass_error"] for x in st) < 1e-6
+
+
 def test_adam_step_matches_oracle():
     """The optimiser step of training (cl_adam) equals the oracle's binary32 Adam bit for
     bit over consecutive steps, moment estimates and bias corrections included."""
