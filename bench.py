@@ -490,9 +490,11 @@ def main():
         out["ensemble"] = run_ensemble(args.variant, args.ensemble_steps, 50)
     if not args.no_extra and not args.no_closure:
         from fip_amd import closure_run
+        # held-out fine grids of 2048^2 and 4096^2 (coarse 512^2 and 1024^2), half each
+        eval_sizes = [2048 if k < (args.closure_test + 1) // 2 else 4096 for k in range(args.closure_test)]
         res = closure_run.experiment(n_train=args.closure_train, n_test=args.closure_test,
                                      duration=args.closure_duration, sample_every=args.closure_every,
-                                     steps=args.closure_steps,
+                                     steps=args.closure_steps, eval_W=eval_sizes,
                                      log=lambda m: print(m, file=sys.stderr, flush=True))
         ev = res.pop("eval")
         # the accuracy target: depth RMSE <= 2 cm and wet-extent IoU >= 0.97
@@ -500,8 +502,9 @@ def main():
         out["closure"] = {
             "workload": (f"{args.closure_train} training scenarios (fine 1024^2 dx=1 m, coarse 256^2 dx=4 m, "
                          f"a pair every {args.closure_every:.0f} s) + {args.closure_test} held-out scenarios "
-                         f"(fine 2048^2 dx=1 m, coarse 512^2 dx=4 m); ridged terrain, discharge inflow, "
-                         f"{args.closure_duration:.0f} s simulated, fixed dt 0.05/0.2 s"),
+                         f"(fine 2048^2 and 4096^2 dx=1 m, coarse 512^2 and 1024^2 dx=4 m, half each, in one "
+                         f"ensemble per model); ridged terrain, discharge inflow, {args.closure_duration:.0f} s "
+                         f"simulated, fixed dt 0.05/0.2 s"),
             "speedup_learned_vs_fine": ev["speedup_learned_vs_fine"],
             "speedup_at_rmse_le_2cm": ev["speedup_learned_vs_fine"] if target_ok else None,
             "training": res, **{k: v for k, v in ev.items() if k not in ("scenarios", "duration_s")}}

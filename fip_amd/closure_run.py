@@ -128,10 +128,14 @@ def generate_samples(cl, scenarios, duration, sample_every=60.0, variant="fast",
 
 def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.65):
     """Plain fine vs plain coarse vs learned coarse on held-out scenarios, as ensembles.
-    Returns metrics and timings."""
+    W: the fine grid size, one for all scenarios or one per scenario (the ensembles then
+    hold scenarios of different sizes); the ridge count scales with the area (200 on
+    2048^2). Returns metrics and timings."""
     from . import metrics as co
     from .sim import Ensemble, tune_ensemble
-    fines = [fine_case(s, W=W, H=W, duration=duration) for s in scenarios]
+    sizes = list(W) if isinstance(W, (list, tuple)) else [W] * len(scenarios)
+    fines = [fine_case(s, W=w, H=w, duration=duration, nridges=max(1, round(200 * (w / 2048.0) ** 2)))
+             for s, w in zip(scenarios, sizes)]
     pairs = [coarse_of(f) for f in fines]
     coarses = [p[0] for p in pairs]
     bstd = np.concatenate([p[1].ravel() for p in pairs])  # each scenario's in turn (sizes may differ)
@@ -192,7 +196,8 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
 def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1500, batch=32, patch=128,
                lr=1e-3, W=1024, eval_W=2048, log=print):
     """Generate samples (fine grids of W^2), train, evaluate on held-out scenarios with
-    fine grids of eval_W^2 (the network is convolutional, so it applies to any size). The
+    fine grids of eval_W^2, or eval_W[k]^2 for held-out scenario k (the network is
+    convolutional, so it applies to any size). The
     held-out samples (four per held-out scenario, one a minute) come first in the store, the
     training samples (one every `sample_every` seconds of each training run) after them."""
     from .sim import Closure
