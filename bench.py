@@ -15,9 +15,10 @@ and kSt read, h, qx, qy written = 32 B with per-cell kSt, 28 B with uniform kSt;
 with HIP events on the simulation's stream; the bytes this build moves per cell, 29 B with
 kSt stored as a one-byte class index, are reported beside it), the masked-mode effective
 rate, the exact-variant rate, the batched ensemble of 64 scenarios of uneven sizes (launch shape from the
-ensemble autotuner), the learned subgrid closure (training on solver-produced pairs,
-then learned coarse vs plain fine ensembles on held-out scenarios: speed and depth RMSE,
-max-depth error, wet-extent IoU) and the CPU oracle on a bounded sample.
+ensemble autotuner), the learned subgrid closure (a GELU and a hard-SiLU network trained
+on the same solver-produced pairs, then learned coarse vs plain fine ensembles on held-out
+scenarios: speed and depth RMSE, max-depth error, wet-extent IoU) and the CPU oracle on a
+bounded sample.
 """
 from __future__ import annotations
 
@@ -374,6 +375,9 @@ def main():
     ap.add_argument("--closure-every", type=float, default=20.0,
                     help="seconds between training pairs of a fine run")
     ap.add_argument("--closure-steps", type=int, default=16000, help="Adam steps")
+    ap.add_argument("--closure-activations", default="gelu,hsilu",
+                    help="one network per activation (gelu, hsilu), trained on the same samples; the first "
+                         "gives the top-level closure figures")
     ap.add_argument("--full-run", action="store_true",
                     help="run the 5667x6500 workload over its whole 12 h from the dry start (masked) "
                          "and print that leg's JSON line only")
@@ -495,10 +499,14 @@ def main():
         res = closure_run.experiment(n_train=args.closure_train, n_test=args.closure_test,
                                      duration=args.closure_duration, sample_every=args.closure_every,
                                      steps=args.closure_steps, eval_W=eval_sizes,
+                                     activations=tuple(args.closure_activations.split(",")),
                                      log=lambda m: print(m, file=sys.stderr, flush=True))
         ev = res.pop("eval")
-        # the accuracy target: depth RMSE <= 2 cm and wet-extent IoU >= 0.97
-        target_ok = ev["learned_depth_rmse_m"] <= 0.02 and ev["learned_wet_iou"] >= 0.97
+        # the accuracy target: depth RMSE <= 2 cm and wet-extent IoU >= 0.97; the fastest
+        # network that meets it
+        nets = [ev] + [ev[a] for a in args.closure_activations.split(",")[1:]]
+        ok = [v["speedup_learned_vs_fine"] for v in nets
+              if v["learned_depth_rmse_m"] <= 0.02 and v["learned_wet_iou"] >= 0.97]
         out["closure"] = {
             "workload": (f"{args.closure_train} training scenarios (fine 1024^2 dx=1 m, coarse 256^2 dx=4 m, "
                          f"a pair every {args.closure_every:.0f} s) + {args.closure_test} held-out scenarios "
@@ -506,7 +514,7 @@ def main():
                          f"ensemble per model); ridged terrain, discharge inflow, {args.closure_duration:.0f} s "
                          f"simulated, fixed dt 0.05/0.2 s"),
             "speedup_learned_vs_fine": ev["speedup_learned_vs_fine"],
-            "speedup_at_rmse_le_2cm": ev["speedup_learned_vs_fine"] if target_ok else None,
+            "speedup_at_rmse_le_2cm": max(ok) if ok else None,
             "training": res, **{k: v for k, v in ev.items() if k not in ("scenarios", "duration_s")}}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -126,11 +126,13 @@ def generate_samples(cl, scenarios, duration, sample_every=60.0, variant="fast",
     log(f"samples: {cl.num_samples()} in {time.perf_counter() - t0:.1f} s")
 
 
-def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.65):
+def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.65, nets=None):
     """Plain fine vs plain coarse vs learned coarse on held-out scenarios, as ensembles.
     W: the fine grid size, one for all scenarios or one per scenario (the ensembles then
     hold scenarios of different sizes); the ridge count scales with the area (200 on
-    2048^2). Returns metrics and timings."""
+    2048^2). nets: networks [(activation, params), ...] that cl runs in turn on the same
+    scenarios (default: cl as it is); the first gives the learned_* figures, each other one
+    a sub-object under its activation's name. Returns metrics and timings."""
     from . import metrics as co
     from .sim import Ensemble, tune_ensemble
     sizes = list(W) if isinstance(W, (list, tuple)) else [W] * len(scenarios)
@@ -144,12 +146,13 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
            "fine": ", ".join(sorted({f"{f.W}x{f.H}" for f in fines})) + " dx=1",
            "coarse": ", ".join(sorted({f"{c.W}x{c.H}" for c in coarses})) + " dx=4"}
 
-    def run(cases, closure):
+    def run(cases, closure, record=True):
         # each ensemble runs with the launch shape the ensemble autotuner picks for it
         best = tune_ensemble(cases, mode="dense", variant=variant, steps=10)
         e = Ensemble(cases, mode="dense", variant=variant, tile_rows=32)
         e.set_launch(best["waves_per_block"], 0)
-        out.setdefault("waves_per_block_fine_plain_learned", []).append(best["waves_per_block"])
+        if record:
+            out.setdefault("waves_per_block_fine_plain_learned", []).append(best["waves_per_block"])
         if closure is not None:
             closure.attach_ensemble(e, bstd, zref, sat=sat)
         e.set_run([duration] * len(cases))
@@ -161,68 +164,100 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
 
     ef, wf = run(fines, None)
     ep, wp = run(coarses, None)
-    el, wl = run(coarses, cl)
-    rm, me, iou = {"plain": [], "learned": []}, {"plain": [], "learned": []}, {"plain": [], "learned": []}
-    fb = 0
-    mass = 0.0
-    for k in range(len(scenarios)):
-        href = block_mean(ef.field(k, "h"))
-        hmref = block_mean(ef.field(k, "hmax"))
-        for name, e in (("plain", ep), ("learned", el)):
+    n = len(scenarios)
+    refs = [(block_mean(ef.field(k, "h")), block_mean(ef.field(k, "hmax"))) for k in range(n)]
+    ef.close()
+    sim_h = n * duration / 3600.0
+
+    def scores(e):
+        rm, me, iou = [], [], []
+        for k, (href, hmref) in enumerate(refs):
             h = e.field(k, "h")
-            rm[name].append(co.depth_rmse(h, href))
-            me[name].append(co.max_depth_error(e.field(k, "hmax"), hmref))
-            iou[name].append(co.wet_iou(h, href))
-        st = cl.status(ens=el, s=k)
-        fb += st["fallback"]
-        mass = max(mass, st["max_rel_mass_error"])
-    sim_h = len(scenarios) * (duration - 0.0) / 3600.0
-    for name, wall in (("fine", wf), ("plain_coarse", wp), ("learned_coarse", wl)):
+            rm.append(co.depth_rmse(h, href))
+            me.append(co.max_depth_error(e.field(k, "hmax"), hmref))
+            iou.append(co.wet_iou(h, href))
+        return {"depth_rmse_m": round(float(np.mean(rm)), 4), "max_depth_err_m": round(float(np.mean(me)), 4),
+                "wet_iou": round(float(np.mean(iou)), 4)}
+
+    for name, wall in (("fine", wf), ("plain_coarse", wp)):
         out[name + "_sim_h_per_wall_s"] = round(sim_h / wall, 3)
         out[name + "_wall_s"] = round(wall, 2)
-    out["speedup_learned_vs_fine"] = round(wf / wl, 2)
-    for name in ("plain", "learned"):
-        out[f"{name}_depth_rmse_m"] = round(float(np.mean(rm[name])), 4)
-        out[f"{name}_max_depth_err_m"] = round(float(np.mean(me[name])), 4)
-        out[f"{name}_wet_iou"] = round(float(np.mean(iou[name])), 4)
-    out["fallback_scenarios"] = fb
-    out["max_rel_mass_error"] = mass
-    for e in (ef, ep, el):
-        e.close()
+    plain = scores(ep)
+    ep.close()
+    for i, (act, params) in enumerate(nets or [(cl.activation, None)]):
+        cl.set_activation(act)
+        if params is not None:
+            cl.params = params
+        el, wl = run(coarses, cl, record=i == 0)
+        learned = scores(el)
+        fb, mass = 0, 0.0
+        for k in range(n):
+            st = cl.status(ens=el, s=k)
+            fb += st["fallback"]
+            mass = max(mass, st["max_rel_mass_error"])
+        el.close()
+        res = {"activation": act, "learned_coarse_sim_h_per_wall_s": round(sim_h / wl, 3),
+               "learned_coarse_wall_s": round(wl, 2), "speedup_learned_vs_fine": round(wf / wl, 2),
+               **{"learned_" + k: v for k, v in learned.items()}, "fallback_scenarios": fb,
+               "max_rel_mass_error": mass}
+        if i == 0:
+            out.update({"learned_activation": act,
+                        "learned_coarse_sim_h_per_wall_s": res["learned_coarse_sim_h_per_wall_s"],
+                        "learned_coarse_wall_s": res["learned_coarse_wall_s"],
+                        "speedup_learned_vs_fine": res["speedup_learned_vs_fine"],
+                        **{"plain_" + k: v for k, v in plain.items()},
+                        **{k: v for k, v in res.items() if k.startswith("learned_") or k in
+                           ("fallback_scenarios", "max_rel_mass_error")}})
+        else:
+            out[act] = {k: v for k, v in res.items() if k != "activation"}
     log(str(out))
     return out
 
 
 def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1500, batch=32, patch=128,
-               lr=1e-3, W=1024, eval_W=2048, log=print):
+               lr=1e-3, W=1024, eval_W=2048, activations=("gelu",), log=print):
     """Generate samples (fine grids of W^2), train, evaluate on held-out scenarios with
     fine grids of eval_W^2, or eval_W[k]^2 for held-out scenario k (the network is
     convolutional, so it applies to any size). The
     held-out samples (four per held-out scenario, one a minute) come first in the store, the
-    training samples (one every `sample_every` seconds of each training run) after them."""
+    training samples (one every `sample_every` seconds of each training run) after them.
+    One network per activation, each trained from the same initial weights on the same
+    samples; the first gives the top-level figures, each other one a sub-object under its
+    name (training figures and the held-out evaluation)."""
     from .sim import Closure
     Wc = W // F
     nsamp = n_train * int(duration // sample_every) + n_test * 4
-    cl = Closure(Wc, Wc, max_samples=nsamp, seed=11)
+    cl = Closure(Wc, Wc, max_samples=nsamp, seed=11, activation=activations[0])
+    p0 = cl.params
     generate_samples(cl, range(1000, 1000 + n_test), 240.0, 60.0, W=W, log=log)
     ntest = cl.num_samples()
     generate_samples(cl, range(n_train), duration, sample_every, W=W, log=log)
-    plain0, learned0 = cl.eval(0, ntest)
-    t0 = time.perf_counter()
-    losses = cl.train(steps, batch=batch, patch=patch, lr=lr, first=ntest)
-    twall = time.perf_counter() - t0
-    plain1, learned1 = cl.eval(0, ntest)
-    res = {"train_samples": cl.num_samples() - ntest, "train_steps": steps, "train_wall_s": round(twall, 2),
-           "loss_first10": float(np.mean(losses[:10])), "loss_last50": float(np.mean(losses[-50:])),
-           "heldout_flux_mse_plain": plain1, "heldout_flux_mse_learned": learned1,
-           "heldout_flux_mse_untrained": learned0}
-    log(str(res))
-    res["eval"] = evaluate(cl, range(1000, 1000 + n_test), duration, W=eval_W, log=log)
+    runs, nets = [], []
+    for act in activations:
+        cl.set_activation(act)
+        cl.params = p0
+        plain0, learned0 = cl.eval(0, ntest)
+        t0 = time.perf_counter()
+        losses = cl.train(steps, batch=batch, patch=patch, lr=lr, first=ntest)
+        twall = time.perf_counter() - t0
+        plain1, learned1 = cl.eval(0, ntest)
+        r = {"train_samples": cl.num_samples() - ntest, "train_steps": steps, "train_wall_s": round(twall, 2),
+             "loss_first10": float(np.mean(losses[:10])), "loss_last50": float(np.mean(losses[-50:])),
+             "heldout_flux_mse_plain": plain1, "heldout_flux_mse_learned": learned1,
+             "heldout_flux_mse_untrained": learned0}
+        log(f"{act}: {r}")
+        runs.append(r)
+        nets.append((act, cl.params))
+    res = dict(runs[0])
+    for act, r in zip(activations[1:], runs[1:]):
+        res[act] = r
+    res["eval"] = evaluate(cl, range(1000, 1000 + n_test), duration, W=eval_W, log=log, nets=nets)
     cl.close()
     return res
 
 
-def time_ensemble_step(steps=300, n=8, W=1024, warm=200, sat=1e30, seed=11, form=None, log=print):
+def time_ensemble_step(steps=300, n=8, W=1024, warm=200, sat=1e30, seed=11, form=None, activation="gelu",
+                       log=print):
     """Per-step wall time of the held-out coarse twins (n scenarios of (W/4)^2) as a plain
     ensemble and with a closure of seeded random weights attached (every face corrected:
     sat = 1e30). Returns (plain_us, learned_us)."""
@@ -232,7 +267,7 @@ def time_ensemble_step(steps=300, n=8, W=1024, warm=200, sat=1e30, seed=11, form
     coarses = [p[0] for p in pairs]
     bstd = np.stack([p[1] for p in pairs])
     zref = np.array([p[2] for p in pairs], np.float32)
-    cl = Closure(W // F, W // F, max_samples=0, seed=seed)
+    cl = Closure(W // F, W // F, max_samples=0, seed=seed, activation=activation)
     if form is not None:
         cl.set_kernel(form)
     out = []
@@ -314,6 +349,7 @@ if __name__ == "__main__":
     ap.add_argument("--warm", type=int, default=200)
     ap.add_argument("--form", type=int, default=None, help="fused kernel form (fip_closure_set_kernel)")
     ap.add_argument("--sat", type=float, default=1e30, help="saturation limit of the network outputs")
+    ap.add_argument("--activation", default="gelu", choices=["gelu", "hsilu"], help="the network's activation")
     ap.add_argument("--drift", action="store_true",
                     help="plain coarse against the restricted fine run over 3 h instead (no closure)")
     a = ap.parse_args()
@@ -322,4 +358,4 @@ if __name__ == "__main__":
     elif a.train_steps:
         time_training(a.train_steps)
     else:
-        time_ensemble_step(a.steps, a.scenarios, a.fine, a.warm, sat=a.sat, form=a.form)
+        time_ensemble_step(a.steps, a.scenarios, a.fine, a.warm, sat=a.sat, form=a.form, activation=a.activation)

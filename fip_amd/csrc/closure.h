@@ -15,7 +15,9 @@
 #define CL_PACKED (2 * CL_TAPS * 64 * 8)   // bf16 elements of one packed layer
 
 typedef unsigned short u16;
-enum { EPI_GELU = 0, EPI_OUT = 1, EPI_DGELU = 2 };
+enum { EPI_ACT = 0, EPI_OUT = 1, EPI_DACT = 2 };
+// the network's activation (include/fip_closure.h: FIP_CLOSURE_ACT_*)
+enum { CL_ACT_GELU = 0, CL_ACT_HSILU = 1 };
 
 // Input features of one coarse state: z - zref, qx, qy, bed - zref, subgrid bed std, kSt.
 struct ClFeat {
@@ -54,16 +56,17 @@ struct ClDom {
 // Network outputs o of every cell of ndom domains (domain d's [H][W][2] image at pixel
 // offset dom[d].o_off; the domains may differ in size, W x H bounds them all). form 2:
 // row march over the work items of cl_fused_plan (items, nitems), 1: persistent tiles,
-// 0: one work-group per tile.
+// 0: one work-group per tile. act: CL_ACT_GELU or CL_ACT_HSILU.
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                           const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems);
+                           const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems,
+                           int act);
 // The row march's work items (domain, first column, first row, end row) for domains of
 // sizes W[d] x H[d] on a device of `cus` compute units: strips of the kernel's width and
 // spans of one length, chosen so that the items (one work-group per compute unit at a
 // time) finish in the fewest march steps.
 std::vector<int4> cl_fused_plan(const int* W, const int* H, int ndom, int cus);
 hipError_t cl_pack_l1_launch(const float* w, u16* wl1, hipStream_t st);
-hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
+hipError_t cl_launch_conv(int epi, int act, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
                           u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,
                           hipStream_t st);
 hipError_t cl_mfma_probe_launch(const u16* A, const u16* B, float* D, hipStream_t st);

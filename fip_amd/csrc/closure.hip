@@ -1,6 +1,6 @@
 // Learned subgrid closure on gfx950: 4x restriction of a fine domain onto a coarse one,
 // the CNN that corrects coarse face discharges (4 layers of 3x3 convolutions, 32
-// channels, GELU, bounded tanh output), its backward pass and Adam.
+// channels, GELU or hard SiLU, bounded tanh output), its backward pass and Adam.
 //
 // Convolutions are implicit GEMMs on the bf16 matrix cores (v_mfma_f32_16x16x32_bf16,
 // fp32 accumulation). Activations are NHWC bf16 with 32 channels (the 6 input features
@@ -52,6 +52,34 @@ __device__ __forceinline__ float dgelu(float x) {
   const float u = s * (x + 0.044715f * x * x * x);
   const float t = fast_tanh(u);
   return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * s * (1.f + 3.f * 0.044715f * x * x);
+}
+
+// Hard SiLU: x * clamp(x / 4 + 1 / 2, 0, 1), x times the logistic function's tangent at 0
+// clamped to [0, 1]: one fused multiply-add with the output clamp and one multiply, no
+// transcendental. x / 4 is exact, so the value is the binary32 rounding of
+// x * RN(x / 4 + 1 / 2) on any IEEE machine. Derivative: 0 for x <= -2, x / 2 + 1 / 2
+// between, 1 for x >= 2.
+__device__ __forceinline__ float hsilu(float x) {
+  return x * fminf(fmaxf(__builtin_fmaf(x, 0.25f, 0.5f), 0.f), 1.f);
+}
+__device__ __forceinline__ float dhsilu(float x) {
+  return x >= 2.f ? 1.f : x <= -2.f ? 0.f : __builtin_fmaf(x, 0.5f, 0.5f);
+}
+
+// the network's activation (CL_ACT_GELU or CL_ACT_HSILU) and its derivative
+template <int ACT>
+__device__ __forceinline__ float act(float x) {
+  if constexpr (ACT == CL_ACT_HSILU)
+    return hsilu(x);
+  else
+    return gelu(x);
+}
+template <int ACT>
+__device__ __forceinline__ float dact(float x) {
+  if constexpr (ACT == CL_ACT_HSILU)
+    return dhsilu(x);
+  else
+    return dgelu(x);
 }
 
 __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, f32x4 c) {
@@ -149,12 +177,12 @@ extern "C" __global__ void cl_gather(const u16* samples, long sample_elems, int 
 // 3x3 convolution (padding 1) of N images H x W, NHWC bf16 with 32 channels, as an
 // implicit GEMM. One wavefront computes 64 pixels of one row x 32 output channels; a
 // work-group holds 4 wavefronts on 4 consecutive rows. Epilogues:
-//   EPI_GELU : Z = acc + bias (bf16), A = GELU(Z) (bf16)
+//   EPI_ACT : Z = acc + bias (bf16), A = act(Z) (bf16)
 //   EPI_OUT  : out = acc + bias, channels 0..1 as fp32 [pixel][2]
-//   EPI_DGELU: out = acc * GELU'(Zp) (bf16)  (backward through the previous layer)
+//   EPI_DACT: out = acc * act'(Zp) (bf16)  (backward through the previous layer)
 // wp: packed weights, [mt 0..1][tap 0..8][lane 0..63][8] bf16 (A fragments).
 
-template <int EPI>
+template <int EPI, int ACT>
 __global__ __launch_bounds__(256) void cl_conv(const u16* __restrict__ X, const u16* __restrict__ wp,
                                                const float* __restrict__ bias, const u16* __restrict__ Zp,
                                                u16* __restrict__ Z, u16* __restrict__ A,
@@ -197,13 +225,13 @@ __global__ __launch_bounds__(256) void cl_conv(const u16* __restrict__ X, const 
       if (x >= W) continue;
       const long pix = ((long)n * H + y) * W + x;
       const int co0 = m * 16 + 4 * grp;
-      if constexpr (EPI == EPI_GELU) {
+      if constexpr (EPI == EPI_ACT) {
         __attribute__((aligned(8))) u16 zb[4], ab[4];
         for (int r = 0; r < 4; r++) {
           const float z = acc[m][t][r] + bias[co0 + r];
           const u16 zq = f2bf(z);
           zb[r] = zq;
-          ab[r] = f2bf(gelu(bf2f(zq)));
+          ab[r] = f2bf(act<ACT>(bf2f(zq)));
         }
         *(uint2*)(Z + pix * CL_C + co0) = *(const uint2*)zb;
         *(uint2*)(A + pix * CL_C + co0) = *(const uint2*)ab;
@@ -216,7 +244,7 @@ __global__ __launch_bounds__(256) void cl_conv(const u16* __restrict__ X, const 
         const uint2 zr = *(const uint2*)(Zp + pix * CL_C + co0);
         const u16* zp = (const u16*)&zr;
         __attribute__((aligned(8))) u16 ob[4];
-        for (int r = 0; r < 4; r++) ob[r] = f2bf(acc[m][t][r] * dgelu(bf2f(zp[r])));
+        for (int r = 0; r < 4; r++) ob[r] = f2bf(acc[m][t][r] * dact<ACT>(bf2f(zp[r])));
         *(uint2*)(Z + pix * CL_C + co0) = *(const uint2*)ob;
       }
     }
@@ -677,7 +705,7 @@ extern "C" __global__ void cl_pack_l1(const float* w, u16* wl1) {
 // One layer of the fused step: wavefront w owns pixel tiles w, w + FNW, ... of the output
 // region (16 pixels each, row-major over the region); for each K chunk it loads the A
 // fragments once and multiplies them into all of its tiles.
-template <int RIN, int ROUT, int MT, bool L1>
+template <int RIN, int ROUT, int MT, bool L1, int ACT>
 __device__ __forceinline__ void fused_layer(const u16* in, u16* out, const u16* wp, const float* bias,
                                             int ox, int oy, int W, int H, float* o2) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -739,18 +767,19 @@ __device__ __forceinline__ void fused_layer(const u16* in, u16* out, const u16* 
 #pragma unroll
     for (int m = 0; m < MT; m++) {
       const int co0 = m * 16 + 4 * grp;
-      // GELU of the fp32 pre-activation (the layer-by-layer form rounds it to bf16 first,
+      // activation of the fp32 pre-activation (the layer-by-layer form rounds it to bf16 first,
       // which it stores for the backward pass; inference has no use for that rounding),
       // packed to bf16 pairs; cells outside the domain are set to 0 by a select
       const uint32_t keep = 0u - (uint32_t)inside;
-      const uint32_t lo = pack_bf2(gelu(acc[i][m][0]), gelu(acc[i][m][1])) & keep;
-      const uint32_t hi = pack_bf2(gelu(acc[i][m][2]), gelu(acc[i][m][3])) & keep;
+      const uint32_t lo = pack_bf2(act<ACT>(acc[i][m][0]), act<ACT>(acc[i][m][1])) & keep;
+      const uint32_t hi = pack_bf2(act<ACT>(acc[i][m][2]), act<ACT>(acc[i][m][3])) & keep;
       *(uint2*)(out + (yy * ROUT + xx) * FPS + co0) = make_uint2(lo, hi);
     }
   }
 }
 
-extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, int par, const u16* wf,
+template <int ACT>
+__global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, int par, const u16* wf,
                                                           const u16* wl1, const float* w, float* o_out) {
   __shared__ __attribute__((aligned(16))) u16 f0[FR0 * FR0 * 8];
   __shared__ __attribute__((aligned(16))) u16 b1[FR1 * FR1 * FPS];
@@ -783,16 +812,16 @@ extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, 
   }
   __syncthreads();
   const float* b0 = w + 0 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C;
-  fused_layer<FR0, FR1, 2, true>(f0, b1, wl1, b0, ox, oy, W, H, nullptr);
+  fused_layer<FR0, FR1, 2, true, ACT>(f0, b1, wl1, b0, ox, oy, W, H, nullptr);
   __syncthreads();
-  fused_layer<FR1, FR2, 2, false>(b1, b2, wf + 1 * CL_PACKED, w + 1 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C, ox,
+  fused_layer<FR1, FR2, 2, false, ACT>(b1, b2, wf + 1 * CL_PACKED, w + 1 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C, ox,
                                   oy, W, H, nullptr);
   __syncthreads();
-  fused_layer<FR2, FR3, 2, false>(b2, b1, wf + 2 * CL_PACKED, w + 2 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C, ox,
+  fused_layer<FR2, FR3, 2, false, ACT>(b2, b1, wf + 2 * CL_PACKED, w + 2 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C, ox,
                                   oy, W, H, nullptr);
   __syncthreads();
   float* o2 = (float*)b2;  // 16 x 16 x 2 outputs reuse the layer-2 buffer
-  fused_layer<FR3, FT, 1, false>(b1, nullptr, wf + 3 * CL_PACKED, w + 3 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C,
+  fused_layer<FR3, FT, 1, false, ACT>(b1, nullptr, wf + 3 * CL_PACKED, w + 3 * CL_LAYER_PARAMS + CL_TAPS * CL_C * CL_C,
                                  ox, oy, W, H, o2);
   __syncthreads();
   // the outputs of the tile's cells into o (one [H][W][2] image per domain)
@@ -827,12 +856,12 @@ extern "C" __global__ __launch_bounds__(512, 2) void cl_fused(const ClDom* dom, 
 // One layer of the persistent fused step: the arithmetic of fused_layer (the same
 // fragments summed in the same order, so the same bits). The wave holds the A fragments
 // of every K chunk of the layer in registers and walks its pixel tiles one after the
-// other; the epilogue (bias-initialised accumulators -> GELU -> bf16 -> LDS) of tile i-1
+// other; the epilogue (bias-initialised accumulators -> activation -> bf16 -> LDS) of tile i-1
 // is issued between the matrix instructions of tile i, so the vector ALU work overlaps
 // the matrix cores. The code is branch-free: the tile count K is a template argument
 // (uniform per wave), and lanes past the region's last pixel store to a spare pixel slot
 // (index NPIX) of the output buffer, or to a per-lane spare slot of o2.
-template <int RIN, int ROUT, int MT, bool L1, int K>
+template <int RIN, int ROUT, int MT, bool L1, int ACT, int K>
 __device__ __forceinline__ void fp_layer_k(const u16* in, u16* out, const u16* wp, const float* bias, int ox,
                                            int oy, int W, int H, float* o2, int wave) {
   const int lane = threadIdx.x & 63;
@@ -883,8 +912,8 @@ __device__ __forceinline__ void fp_layer_k(const u16* in, u16* out, const u16* w
 #pragma unroll
     for (int m = 0; m < MT; m++) {
       const int co0 = m * 16 + 4 * grp;
-      const uint32_t lo = pack_bf2(gelu(acc[m][0]), gelu(acc[m][1])) & keep;
-      const uint32_t hi = pack_bf2(gelu(acc[m][2]), gelu(acc[m][3])) & keep;
+      const uint32_t lo = pack_bf2(act<ACT>(acc[m][0]), act<ACT>(acc[m][1])) & keep;
+      const uint32_t hi = pack_bf2(act<ACT>(acc[m][2]), act<ACT>(acc[m][3])) & keep;
       *(uint2*)(out + px * FPS + co0) = make_uint2(lo, hi);
     }
   };
@@ -911,16 +940,16 @@ __device__ __forceinline__ void fp_layer_k(const u16* in, u16* out, const u16* w
 }
 
 // tiles are dealt round-robin, so a wave has TPW or TPW - 1 of them
-template <int RIN, int ROUT, int MT, bool L1>
+template <int RIN, int ROUT, int MT, bool L1, int ACT>
 __device__ __forceinline__ void fp_layer(const u16* in, u16* out, const u16* wp, const float* bias, int ox, int oy,
                                          int W, int H, float* o2) {
   constexpr int NT = (ROUT * ROUT + 15) / 16, TPW = (NT + FNW - 1) / FNW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nt = (NT - wave + FNW - 1) / FNW;
   if (nt == TPW)
-    fp_layer_k<RIN, ROUT, MT, L1, TPW>(in, out, wp, bias, ox, oy, W, H, o2, wave);
+    fp_layer_k<RIN, ROUT, MT, L1, ACT, TPW>(in, out, wp, bias, ox, oy, W, H, o2, wave);
   else if constexpr (TPW > 1)
-    fp_layer_k<RIN, ROUT, MT, L1, TPW - 1>(in, out, wp, bias, ox, oy, W, H, o2, wave);
+    fp_layer_k<RIN, ROUT, MT, L1, ACT, TPW - 1>(in, out, wp, bias, ox, oy, W, H, o2, wave);
 }
 
 struct FpTile {   // one tile of the persistent loop (wave-uniform)
@@ -1004,7 +1033,8 @@ __device__ __forceinline__ void fp_store(const FpDom& D, FpTile T, const FpCells
   }
 }
 
-extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_eu(2, 2))) void cl_fused_p(
+template <int ACT>
+__global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_eu(2, 2))) void cl_fused_p(
     const ClDom* __restrict__ dom, int par, const u16* wf, const u16* wl1, const float* w, float* o_out, int tx,
     int ty, int ntiles, int per_wg) {
   __shared__ __attribute__((aligned(16))) u16 sw[FP_WN];
@@ -1039,18 +1069,18 @@ extern "C" __global__ __launch_bounds__(512, 1) __attribute__((amdgpu_waves_per_
     // domain descriptor of the tile after it is read (scalar loads, used next iteration)
     if (tn < end) fp_load(DN, N, pf);  // in flight during layers 0 and 1
     const int W = DT.W, H = DT.H;
-    fp_layer<FR0, FR1, 2, true>(f0, b1, sw, sb, T.ox, T.oy, W, H, nullptr);
+    fp_layer<FR0, FR1, 2, true, ACT>(f0, b1, sw, sb, T.ox, T.oy, W, H, nullptr);
     const int tnn = tn < end ? fp_next(dom, tn + 1, end, tx, ty, par) : end;
     const FpTile NN = fp_tile(tnn < end ? tnn : (tn < end ? tn : t), tx, ty);
     const FpDom DNN = fp_dom(dom, NN.d, cur);
     __syncthreads();
-    fp_layer<FR1, FR2, 2, false>(b1, b2, sw + FP_W1, sb + CL_C, T.ox, T.oy, W, H, nullptr);
+    fp_layer<FR1, FR2, 2, false, ACT>(b1, b2, sw + FP_W1, sb + CL_C, T.ox, T.oy, W, H, nullptr);
     if (tn < end) fp_store(DN, N, pf, f0);
     __syncthreads();
-    fp_layer<FR2, FR3, 2, false>(b2, b1, sw + FP_W2, sb + 2 * CL_C, T.ox, T.oy, W, H, nullptr);
+    fp_layer<FR2, FR3, 2, false, ACT>(b2, b1, sw + FP_W2, sb + 2 * CL_C, T.ox, T.oy, W, H, nullptr);
     __syncthreads();
     float* o2 = (float*)b2;
-    fp_layer<FR3, FT, 1, false>(b1, nullptr, sw + FP_W3, sb + 3 * CL_C, T.ox, T.oy, W, H, o2);
+    fp_layer<FR3, FT, 1, false, ACT>(b1, nullptr, sw + FP_W3, sb + 3 * CL_C, T.ox, T.oy, W, H, o2);
     __syncthreads();
     const int e = threadIdx.x;
     const int x = T.ox + e % FT, y = T.oy + e / FT;
@@ -1137,9 +1167,9 @@ __device__ __forceinline__ void rm_mfma2_l0(const u16* in, const int (&off)[3][R
   }
 }
 
-// GELU epilogue of one layer row into its ring (slot S0): region column c of a region of
+// activation epilogue of one layer row into its ring (slot S0): region column c of a region of
 // width RW; lanes past the region store to the spare slot
-template <int RW, int S0>
+template <int RW, int S0, int ACT>
 __device__ __forceinline__ void rm_store(u16* out, int y, int c, int x, int W, int H, const f32x4 (&acc)[2],
                                          int grp) {
   const bool valid = c < RW;
@@ -1149,8 +1179,8 @@ __device__ __forceinline__ void rm_store(u16* out, int y, int c, int x, int W, i
 #pragma unroll
   for (int m = 0; m < 2; m++) {
     const int co0 = m * 16 + 4 * grp;
-    const uint32_t lo = pack_bf2(gelu(acc[m][0]), gelu(acc[m][1])) & keep;
-    const uint32_t hi = pack_bf2(gelu(acc[m][2]), gelu(acc[m][3])) & keep;
+    const uint32_t lo = pack_bf2(act<ACT>(acc[m][0]), act<ACT>(acc[m][1])) & keep;
+    const uint32_t hi = pack_bf2(act<ACT>(acc[m][2]), act<ACT>(acc[m][3])) & keep;
     *(uint2*)(out + (S0 * RPX + px) * FPS + co0) = make_uint2(lo, hi);
   }
 }
@@ -1170,7 +1200,7 @@ struct RmRow {
 // One wavefront of layer L (0..3) on pixel tile j over the march steps: reads its input
 // ring `in`, writes its output ring `out` (layer 3: the outputs o; it also loads the
 // features of the next rows and writes them to `out`, the feature ring). Layer L computes rows ya - M + 2k and the next at step k (M = 8, 14, 18, 24).
-template <int L>
+template <int L, int ACT>
 __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u16* in, u16* out, int j, int lane,
                                          int x0, int ya, int nrows, int W, int H, float* od, const ClDom& D,
                                          int cur) {
@@ -1245,8 +1275,8 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
     constexpr int S = decltype(slot)::value;
     const int y = ya - M + 2 * k;
     if constexpr (L < 3) {
-      rm_store<RW, S>(out, y, c, x0 + c - HALO, W, H, acc0, grp);
-      rm_store<RW, rm_slot(S + 1)>(out, y + 1, c, x0 + c - HALO, W, H, acc1, grp);
+      rm_store<RW, S, ACT>(out, y, c, x0 + c - HALO, W, H, acc0, grp);
+      rm_store<RW, rm_slot(S + 1), ACT>(out, y + 1, c, x0 + c - HALO, W, H, acc1, grp);
     } else {  // outputs 0 and 1 (lane group 0) of rows y and y + 1 that the span owns
       const int x = x0 + c;
       if (grp == 0 && c < RS && x < W) {
@@ -1326,7 +1356,8 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
   }
 }
 
-extern "C" __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __restrict__ dom, int par,
+template <int ACT>
+__global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __restrict__ dom, int par,
                                                              const u16* wf, const u16* wl1, const float* w,
                                                              float* o_out, const int4* __restrict__ items) {
   __shared__ __attribute__((aligned(16))) u16 fring[RROWS * RFW * 8];
@@ -1346,14 +1377,14 @@ extern "C" __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __
   float* od = o_out + D.o_off * 2;
   const float* bias = w + CL_TAPS * CL_C * CL_C;  // + layer * CL_LAYER_PARAMS
   if (layer == 0)
-    rm_march<0>(wl1, bias, fring, ring1, j, lane, x0, ya, nrows, W, H, od, D, cur);
+    rm_march<0, ACT>(wl1, bias, fring, ring1, j, lane, x0, ya, nrows, W, H, od, D, cur);
   else if (layer == 1)
-    rm_march<1>(wf + CL_PACKED, bias + CL_LAYER_PARAMS, ring1, ring2, j, lane, x0, ya, nrows, W, H, od, D, cur);
+    rm_march<1, ACT>(wf + CL_PACKED, bias + CL_LAYER_PARAMS, ring1, ring2, j, lane, x0, ya, nrows, W, H, od, D, cur);
   else if (layer == 2)
-    rm_march<2>(wf + 2 * CL_PACKED, bias + 2 * CL_LAYER_PARAMS, ring2, ring3, j, lane, x0, ya, nrows, W, H, od, D,
+    rm_march<2, ACT>(wf + 2 * CL_PACKED, bias + 2 * CL_LAYER_PARAMS, ring2, ring3, j, lane, x0, ya, nrows, W, H, od, D,
                 cur);
   else
-    rm_march<3>(wf + 3 * CL_PACKED, bias + 3 * CL_LAYER_PARAMS, ring3, fring, j, lane, x0, ya, nrows, W, H, od, D,
+    rm_march<3, ACT>(wf + 3 * CL_PACKED, bias + 3 * CL_LAYER_PARAMS, ring3, fring, j, lane, x0, ya, nrows, W, H, od, D,
                 cur);
 }
 
@@ -1409,21 +1440,31 @@ extern "C" __global__ void cl_feat_from_float(const float* f, long npix, u16* ou
 
 // ---------------------------------------------------------------------------------------
 // launchers (host side of this translation unit)
-hipError_t cl_launch_conv(int epi, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
+template <int ACT>
+void launch_conv(int epi, dim3 grid, dim3 block, const u16* X, const u16* wp, const float* bias, const u16* Zp,
+                 u16* Z, u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par, hipStream_t st) {
+  switch (epi) {
+    case EPI_ACT:
+      hipLaunchKernelGGL((cl_conv<EPI_ACT, ACT>), grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par);
+      break;
+    case EPI_OUT:
+      hipLaunchKernelGGL((cl_conv<EPI_OUT, ACT>), grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par);
+      break;
+    default:
+      hipLaunchKernelGGL((cl_conv<EPI_DACT, ACT>), grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl,
+                         par);
+      break;
+  }
+}
+
+hipError_t cl_launch_conv(int epi, int act, const u16* X, const u16* wp, const float* bias, const u16* Zp, u16* Z,
                           u16* A, float* out, int N, int H, int W, const fip_ctl* ctl, int par,
                           hipStream_t st) {
   const dim3 grid((W + 63) / 64, (H + 3) / 4, N), block(256);
-  switch (epi) {
-    case EPI_GELU:
-      hipLaunchKernelGGL(cl_conv<EPI_GELU>, grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par);
-      break;
-    case EPI_OUT:
-      hipLaunchKernelGGL(cl_conv<EPI_OUT>, grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par);
-      break;
-    default:
-      hipLaunchKernelGGL(cl_conv<EPI_DGELU>, grid, block, 0, st, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par);
-      break;
-  }
+  if (act == CL_ACT_HSILU)
+    launch_conv<CL_ACT_HSILU>(epi, grid, block, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par, st);
+  else
+    launch_conv<CL_ACT_GELU>(epi, grid, block, X, wp, bias, Zp, Z, A, out, N, H, W, ctl, par, st);
   return hipGetLastError();
 }
 
@@ -1509,16 +1550,17 @@ std::vector<int4> cl_fused_plan(const int* W, const int* H, int ndom, int cus) {
   return out;
 }
 
-hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                           const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems) {
+template <int ACT>
+hipError_t fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
+                        const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems) {
   const int tx = (W + FT - 1) / FT, ty = (H + FT - 1) / FT;
   if (form == 0) {
-    hipLaunchKernelGGL(cl_fused, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, o);
+    hipLaunchKernelGGL(cl_fused<ACT>, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, o);
     return hipGetLastError();
   }
   if (form == 2) {
     if (!items || nitems < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(cl_fused_r, dim3(nitems), dim3(64 * RNW), 0, st, dom, par, wf, wl1, w, o, items);
+    hipLaunchKernelGGL(cl_fused_r<ACT>, dim3(nitems), dim3(64 * RNW), 0, st, dom, par, wf, wl1, w, o, items);
     return hipGetLastError();
   }
   static int cus[64] = {};
@@ -1534,6 +1576,14 @@ hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, co
   const int ntiles = tx * ty * ndom;
   const int per = (ntiles + cus[dev] - 1) / cus[dev];
   const int groups = (ntiles + per - 1) / per;
-  hipLaunchKernelGGL(cl_fused_p, dim3(groups), dim3(512), 0, st, dom, par, wf, wl1, w, o, tx, ty, ntiles, per);
+  hipLaunchKernelGGL(cl_fused_p<ACT>, dim3(groups), dim3(512), 0, st, dom, par, wf, wl1, w, o, tx, ty, ntiles, per);
   return hipGetLastError();
+}
+
+hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
+                           const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems,
+                           int act) {
+  if (act == CL_ACT_HSILU)
+    return fused_launch<CL_ACT_HSILU>(dom, ndom, W, H, par, wf, wl1, w, o, st, form, items, nitems);
+  return fused_launch<CL_ACT_GELU>(dom, ndom, W, H, par, wf, wl1, w, o, st, form, items, nitems);
 }

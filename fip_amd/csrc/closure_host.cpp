@@ -17,6 +17,9 @@
 
 using fip_internal::fail;
 
+static_assert(CL_ACT_GELU == FIP_CLOSURE_ACT_GELU && CL_ACT_HSILU == FIP_CLOSURE_ACT_HSILU,
+              "activation codes of the C ABI and the kernels");
+
 #define CHIP(expr)                                                                          \
   do {                                                                                      \
     hipError_t e_ = (expr);                                                                 \
@@ -167,6 +170,7 @@ struct fip_closure {
   float* part = nullptr;
   int t = 0;            // Adam steps taken
   int fused_form = 2;   // launch form of the fused inference kernel (cl_fused_launch)
+  int act = CL_ACT_GELU;  // the network's activation
   // training samples
   int Wc = 0, Hc = 0, cap = 0, ns = 0;
   u16* s_feat = nullptr;     // [cap][Hc][Wc][8]
@@ -193,12 +197,12 @@ int pack(fip_closure* cl) {
 int forward(fip_closure* cl, Work& ws, const fip_ctl* ctl, int par, hipStream_t st) {
   const u16* in = ws.x0;
   for (int l = 0; l < 3; l++) {
-    CHIP(cl_launch_conv(EPI_GELU, in, cl->wf + (size_t)l * CL_PACKED, bias_of(cl, l), nullptr, ws.z[l], ws.a[l],
-                        nullptr, ws.N, ws.H, ws.W, ctl, par, st));
+    CHIP(cl_launch_conv(EPI_ACT, cl->act, in, cl->wf + (size_t)l * CL_PACKED, bias_of(cl, l), nullptr, ws.z[l],
+                        ws.a[l], nullptr, ws.N, ws.H, ws.W, ctl, par, st));
     in = ws.a[l];
   }
-  CHIP(cl_launch_conv(EPI_OUT, in, cl->wf + 3 * (size_t)CL_PACKED, bias_of(cl, 3), nullptr, nullptr, nullptr, ws.o,
-                      ws.N, ws.H, ws.W, ctl, par, st));
+  CHIP(cl_launch_conv(EPI_OUT, cl->act, in, cl->wf + 3 * (size_t)CL_PACKED, bias_of(cl, 3), nullptr, nullptr,
+                      nullptr, ws.o, ws.N, ws.H, ws.W, ctl, par, st));
   return FIP_OK;
 }
 
@@ -213,8 +217,8 @@ int backward(fip_closure* cl, Work& ws, float bound, hipStream_t st) {
     CHIP(cl_wgrad_launch(xin[l], ws.dz[l], cl->part, ws.N, ws.H, ws.W, nb, st));
     CHIP(cl_reduce_launch(cl->part, nb, CL_LAYER_PARAMS, cl->g + (size_t)l * CL_LAYER_PARAMS, st));
     if (l > 0)
-      CHIP(cl_launch_conv(EPI_DGELU, ws.dz[l], cl->wb + (size_t)l * CL_PACKED, nullptr, ws.z[l - 1], ws.dz[l - 1],
-                          nullptr, nullptr, ws.N, ws.H, ws.W, nullptr, 0, st));
+      CHIP(cl_launch_conv(EPI_DACT, cl->act, ws.dz[l], cl->wb + (size_t)l * CL_PACKED, nullptr, ws.z[l - 1],
+                          ws.dz[l - 1], nullptr, nullptr, ws.N, ws.H, ws.W, nullptr, 0, st));
   }
   return FIP_OK;
 }
@@ -582,7 +586,7 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
     const fip_grid& g = fip_internal::grid(ap->sim);
     CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form, ap->items,
-                         ap->nitems));
+                         ap->nitems, cl->act));
     CHIP(cl_correct_launch(g, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, st));
     CHIP(cl_mass_check_launch(g, par, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
@@ -637,7 +641,7 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
   fip_internal::ens_set_post_hook(ens, [cl, ap, n](hipStream_t st, int par) -> int {
     const fip_ens_scn* scn = fip_internal::ens_scn(ap->ens);
     CHIP(cl_fused_launch(ap->dom, n, ap->wmax, ap->hmax, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form,
-                         ap->items, ap->nitems));
+                         ap->items, ap->nitems, cl->act));
     CHIP(cl_correct_ens_launch(scn, ap->dom, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, ap->wmax,
                                ap->hmax, n, st));
     CHIP(cl_mass_check_ens_launch(scn, n, par, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
@@ -711,12 +715,12 @@ int fip_closure_forward_noalloc(fip_closure* cl, const float* feat, int N, int H
   CHIP(cl_feat_from_float_launch(feat, npix, x0, cl->stream));
   const u16* in = x0;
   for (int l = 0; l < 3; l++) {
-    CHIP(cl_launch_conv(EPI_GELU, in, cl->wf + (size_t)l * CL_PACKED, bias_of(cl, l), nullptr, z, a[l & 1],
-                        nullptr, N, H, W, nullptr, 0, cl->stream));
+    CHIP(cl_launch_conv(EPI_ACT, cl->act, in, cl->wf + (size_t)l * CL_PACKED, bias_of(cl, l), nullptr, z,
+                        a[l & 1], nullptr, N, H, W, nullptr, 0, cl->stream));
     in = a[l & 1];
   }
-  CHIP(cl_launch_conv(EPI_OUT, in, cl->wf + 3 * (size_t)CL_PACKED, bias_of(cl, 3), nullptr, nullptr, nullptr, out,
-                      N, H, W, nullptr, 0, cl->stream));
+  CHIP(cl_launch_conv(EPI_OUT, cl->act, in, cl->wf + 3 * (size_t)CL_PACKED, bias_of(cl, 3), nullptr, nullptr,
+                      nullptr, out, N, H, W, nullptr, 0, cl->stream));
   CHIP(hipStreamSynchronize(cl->stream));
   return FIP_OK;
 }
@@ -769,11 +773,26 @@ int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed
   // par such that par ^ 1 is the current side and the halt flag read is that of no step
   const int par = fip_internal::cur_side(coarse) ^ 1;
   hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, o, nullptr, cl->fused_form, items,
-                                 nitems);
+                                 nitems, cl->act);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(out, o, spx * 2 * sizeof(float), hipMemcpyDeviceToHost);
   hipFree(bstd); hipFree(o); hipFree(dom); hipFree(items);
   if (e != hipSuccess) return fail(FIP_E_HIP, std::string("fused outputs: ") + hipGetErrorString(e));
+  return FIP_OK;
+}
+
+int fip_closure_set_activation(fip_closure* cl, int act) {
+  if (!cl) return fail(FIP_E_INVALID, "null argument");
+  if (act != FIP_CLOSURE_ACT_GELU && act != FIP_CLOSURE_ACT_HSILU)
+    return fail(FIP_E_INVALID, "activation must be 0 (GELU) or 1 (hard SiLU)");
+  if (!cl->att.empty()) return fail(FIP_E_INVALID, "set the activation before attaching the closure");
+  cl->act = act;
+  return FIP_OK;
+}
+
+int fip_closure_get_activation(fip_closure* cl, int* act) {
+  if (!cl || !act) return fail(FIP_E_INVALID, "null argument");
+  *act = cl->act;
   return FIP_OK;
 }
 

@@ -27,7 +27,8 @@ CLOSURE_EXPORTS = ["fip_restrict", "fip_closure_create", "fip_closure_destroy", 
                    "fip_closure_train", "fip_closure_adam_step", "fip_closure_eval", "fip_closure_attach",
                    "fip_closure_attach_ensemble", "fip_closure_detach", "fip_closure_status",
                    "fip_closure_forward", "fip_closure_forward_noalloc", "fip_closure_backward", "fip_closure_fused_outputs",
-                   "fip_closure_set_kernel", "fip_mfma_probe"]
+                   "fip_closure_set_kernel", "fip_closure_set_activation", "fip_closure_get_activation",
+                   "fip_mfma_probe"]
 CL_PARAMS = 4 * (9 * 32 * 32 + 32)
 ENS_EXPORTS = ["fip_ens_create", "fip_ens_destroy", "fip_ens_set_bcs", "fip_ens_set_run",
                "fip_ens_simulate", "fip_ens_prepare", "fip_ens_get_time", "fip_ens_download", "fip_ens_set_launch",
@@ -136,6 +137,8 @@ def lib():
                                            C.POINTER(C.c_ubyte), C.c_float, fp, dp]
         L.fip_closure_fused_outputs.argtypes = [vp, vp, fp, C.c_float, fp]
         L.fip_closure_set_kernel.argtypes = [vp, C.c_int]
+        L.fip_closure_set_activation.argtypes = [vp, C.c_int]
+        L.fip_closure_get_activation.argtypes = [vp, C.POINTER(C.c_int)]
         L.fip_mfma_probe.argtypes = [C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), fp]
         L.fip_check_rcp.argtypes = [C.c_int, C.c_float, C.c_float, C.POINTER(C.c_ulonglong), fp]
         L.fip_tune.argtypes = [C.POINTER(FipDesc), C.POINTER(FipBC), C.c_int, C.c_int,
@@ -485,14 +488,35 @@ def mfma_probe(A, B, device=0):
     return D
 
 
-class Closure:
-    """The learned subgrid closure (include/fip_closure.h)."""
+ACTIVATIONS = {"gelu": 0, "hsilu": 1}  # FIP_CLOSURE_ACT_GELU, FIP_CLOSURE_ACT_HSILU
 
-    def __init__(self, Wc, Hc, max_samples=0, seed=1, device=0):
+
+class Closure:
+    """The learned subgrid closure (include/fip_closure.h). activation: "gelu" (default) or
+    "hsilu" (hard SiLU, x * clamp(x / 4 + 1 / 2, 0, 1))."""
+
+    def __init__(self, Wc, Hc, max_samples=0, seed=1, device=0, activation="gelu"):
         h = C.c_void_p()
         _check(lib().fip_closure_create(device, seed, Wc, Hc, max_samples, C.byref(h)))
         self.c = h
         self.Wc, self.Hc = Wc, Hc
+        try:
+            self.set_activation(activation)
+        except Exception:
+            self.close()
+            raise
+
+    @property
+    def activation(self) -> str:
+        a = C.c_int()
+        _check(lib().fip_closure_get_activation(self.c, C.byref(a)))
+        return {v: k for k, v in ACTIVATIONS.items()}[a.value]
+
+    def set_activation(self, activation: str):
+        """Switch the network's activation (not while attached); the parameters stay."""
+        if activation not in ACTIVATIONS:
+            raise ValueError(f"activation must be one of {sorted(ACTIVATIONS)}")
+        _check(lib().fip_closure_set_activation(self.c, ACTIVATIONS[activation]))
 
     def close(self):
         if getattr(self, "c", None):
@@ -577,7 +601,8 @@ class Closure:
         _check(lib().fip_closure_forward_noalloc(self.c, C.c_void_p(feat_dev), N, H, W, C.c_void_p(out_dev)))
 
     def set_kernel(self, form: int):
-        """Launch form of the fused inference kernel: 1 persistent (default), 0 per tile."""
+        """Launch form of the fused inference kernel: 2 row march (default), 1 persistent
+        tiles, 0 one work-group per tile."""
         _check(lib().fip_closure_set_kernel(self.c, int(form)))
 
     def fused_outputs(self, coarse: "Sim", bed_std, zref):

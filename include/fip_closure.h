@@ -1,11 +1,11 @@
 /* C ABI of libfiphip: the learned subgrid closure that lets the solver run on a grid
  * `factor` times coarser (factor 4: dx 1 m -> 4 m).
  *
- * A small CNN (4 layers of 3x3 convolutions, 32 channels, GELU) reads six channels of a
- * coarse state (z, qx, qy, bed, subgrid bed std, kSt) and outputs two corrections of the
- * coarse face discharges, c = tanh(o) * 0.1 * |q|. Corrections are added to interior
- * faces only, so water only moves between cells of the domain: mass is conserved by
- * construction. It is trained on the GPU (bf16 matrix-core convolutions, fp32
+ * A small CNN (4 layers of 3x3 convolutions, 32 channels, GELU or hard SiLU) reads six
+ * channels of a coarse state (z, qx, qy, bed, subgrid bed std, kSt) and outputs two
+ * corrections of the coarse face discharges, c = tanh(o) * 0.1 * |q|. Corrections are
+ * added to interior faces only, so water only moves between cells of the domain: mass is
+ * conserved by construction. It is trained on the GPU (bf16 matrix-core convolutions, fp32
  * accumulation, Adam) on pairs the solver itself produces: a restricted fine state, one
  * plain coarse step from it, and the restricted fine state one coarse step later.
  *
@@ -43,6 +43,16 @@ int fip_restrict(fip_sim* fine, fip_sim* coarse, int factor);
 int fip_closure_create(int device, unsigned long long seed, int Wc, int Hc, int max_samples,
                        fip_closure** out);
 int fip_closure_destroy(fip_closure* cl);
+/* The activation after layers 0..2. GELU (default): tanh form,
+ * x / (1 + 2^(-2 sqrt(2/pi) log2(e) x (1 + 0.044715 x^2))). Hard SiLU:
+ * x * clamp(x / 4 + 1 / 2, 0, 1), no transcendental instruction, so the fused inference
+ * kernel is faster; trained networks of the two differ in accuracy. Training, inference
+ * and gradients all use it; parameters trained with one are meant for that one. Set it
+ * before fip_closure_attach*. */
+#define FIP_CLOSURE_ACT_GELU 0
+#define FIP_CLOSURE_ACT_HSILU 1
+int fip_closure_set_activation(fip_closure* cl, int act);
+int fip_closure_get_activation(fip_closure* cl, int* act);
 int fip_closure_get_params(fip_closure* cl, float* params);
 int fip_closure_set_params(fip_closure* cl, const float* params);
 

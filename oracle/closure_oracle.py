@@ -7,8 +7,9 @@ reference vectors: the restriction by closed forms and conservation invariants, 
 network's backward pass by central finite differences in binary64
 (tests/test_closure_oracle.py).
 
-Network: 4 layers of 3x3 convolutions with zero padding, 32 channels, GELU (tanh form)
-after the first three; inputs 6 features, outputs 2 (corrections of qx and qy). Parameters
+Network: 4 layers of 3x3 convolutions with zero padding, 32 channels, an activation after
+the first three (act="gelu": GELU, tanh form; act="hsilu": hard SiLU,
+x * clamp(x / 4 + 1 / 2, 0, 1)); inputs 6 features, outputs 2 (corrections of qx and qy). Parameters
 in the padded layout of include/fip_closure.h: per layer W[tap][ci][co] (32 x 32) then
 bias[co]; tap t = (dy + 1) * 3 + (dx + 1) reads the input at (y + dy, x + dx).
 
@@ -49,6 +50,24 @@ def dgelu(x):
     u = S * (x + 0.044715 * x ** 3)
     t = np.tanh(u)
     return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * S * (1.0 + 3.0 * 0.044715 * x * x)
+
+
+def hsilu(x):
+    """Hard SiLU, x * clamp(x / 4 + 1 / 2, 0, 1), in the dtype of x (x / 4 is exact, so in
+    float32 this is the GPU's fused multiply-add with clamp and multiply, bit for bit)."""
+    x = np.asarray(x)
+    t = np.clip(x * x.dtype.type(0.25) + x.dtype.type(0.5), 0, 1).astype(x.dtype)
+    return x * t
+
+
+def dhsilu(x):
+    """Derivative of hsilu: 0 for x <= -2, x / 2 + 1 / 2 between, 1 for x >= 2."""
+    x = np.asarray(x)
+    mid = x * x.dtype.type(0.5) + x.dtype.type(0.5)
+    return np.where(x >= 2, x.dtype.type(1), np.where(x <= -2, x.dtype.type(0), mid)).astype(x.dtype)
+
+
+ACTS = {"gelu": (gelu, dgelu), "hsilu": (hsilu, dhsilu)}
 
 
 def split(params):
@@ -114,8 +133,9 @@ def pad_features(feat):
     return f
 
 
-def forward(feat, params, emulate=True):
+def forward(feat, params, emulate=True, act="gelu"):
     """feat [N, H, W, 6] -> raw outputs o [N, H, W, 2] and the cache for backward."""
+    f_act = ACTS[act][0]
     layers = split(params)
     acc = np.float32 if emulate else np.float64
     rnd = bf16 if emulate else (lambda a: a)
@@ -124,7 +144,7 @@ def forward(feat, params, emulate=True):
     for l in range(LAYERS - 1):
         w, b = layers[l]
         z = rnd(conv(x, rnd(w.astype(acc)), b.astype(acc), acc=acc).astype(acc))
-        a = rnd(gelu(z).astype(acc))
+        a = rnd(f_act(z).astype(acc))
         cache["z"].append(z)
         cache["x"].append(a)
         x = a
@@ -141,9 +161,10 @@ def loss_terms(o, qp, qt, mask, bound):
     return th, sc, r, m
 
 
-def loss_and_grad(feat, params, qp, qt, mask, bound=0.1, emulate=True):
+def loss_and_grad(feat, params, qp, qt, mask, bound=0.1, emulate=True, act="gelu"):
     """Loss sum m r^2 / (2 N H W) and its gradient in the padded parameter layout."""
-    o, cache = forward(feat, params, emulate)
+    o, cache = forward(feat, params, emulate, act)
+    d_act = ACTS[act][1]
     acc = np.float32 if emulate else np.float64
     rnd = bf16 if emulate else (lambda a: a)
     N, H, W, _ = o.shape
@@ -163,7 +184,7 @@ def loss_and_grad(feat, params, qp, qt, mask, bound=0.1, emulate=True):
         grads.append((dw, db))
         if l > 0:
             dx = conv_input_grad(dz, rnd(w.astype(acc)), acc=acc)
-            dz = rnd((dx * dgelu(cache["z"][l - 1].astype(acc))).astype(acc))
+            dz = rnd((dx * d_act(cache["z"][l - 1].astype(acc))).astype(acc))
     grads = grads[::-1]
     g = join(grads, dtype=np.float64)
     return loss, g

@@ -35,26 +35,29 @@ def test_mfma_operand_layout():
     assert np.abs(D - ref).max() < 1e-4 * np.abs(ref).max(), np.abs(D - ref).max()
 
 
+@pytest.mark.parametrize("act", ["gelu", "hsilu"])
 @pytest.mark.parametrize("N,H,W", [(1, 9, 20), (2, 20, 70), (1, 33, 128)])
-def test_forward_matches_oracle(N, H, W):
+def test_forward_matches_oracle(N, H, W, act):
     from fip_amd.sim import Closure
     rng = np.random.default_rng(N * 100 + H)
-    c = Closure(16, 16, 0, seed=3)
+    c = Closure(16, 16, 0, seed=3, activation=act)
+    assert c.activation == act
     p = rand_params(rng)
     c.params = p
     feat = rng.normal(0, 1, (N, H, W, co.FEAT)).astype(np.float32)
     got = c.forward(feat)
-    ref, _ = co.forward(feat, p, emulate=True)
+    ref, _ = co.forward(feat, p, emulate=True, act=act)
     err = np.abs(got - ref).max()
     assert err <= 2e-3 * max(1.0, np.abs(ref).max()), err
     c.close()
 
 
-def test_backward_matches_oracle():
+@pytest.mark.parametrize("act", ["gelu", "hsilu"])
+def test_backward_matches_oracle(act):
     from fip_amd.sim import Closure
     rng = np.random.default_rng(11)
     N, H, W = 2, 24, 70
-    c = Closure(16, 16, 0, seed=3)
+    c = Closure(16, 16, 0, seed=3, activation=act)
     p = rand_params(rng)
     c.params = p
     feat = rng.normal(0, 1, (N, H, W, co.FEAT)).astype(np.float32)
@@ -62,7 +65,7 @@ def test_backward_matches_oracle():
     qt = (qp * (1 + rng.normal(0, 0.05, qp.shape))).astype(np.float32)
     mask = (rng.uniform(0, 1, qp.shape) > 0.1).astype(np.uint8)
     loss, g = c.backward(feat, qp, qt, mask)
-    rloss, rg = co.loss_and_grad(feat, p, qp, qt, mask, emulate=True)
+    rloss, rg = co.loss_and_grad(feat, p, qp, qt, mask, emulate=True, act=act)
     assert abs(loss - rloss) <= 1e-4 * abs(rloss), (loss, rloss)
     for l in range(co.LAYERS):
         a = g[l * co.LP:(l + 1) * co.LP].astype(np.float64)
@@ -183,16 +186,17 @@ def test_inference_hook_mass_and_fallback():
     big.close()
 
 
-def test_fused_step_matches_layerwise_forward():
+@pytest.mark.parametrize("act", ["gelu", "hsilu"])
+def test_fused_step_matches_layerwise_forward(act):
     """One coarse step with the closure attached equals the plain step plus
     tanh(o) 0.1 |q| on interior faces, o from the layer-by-layer network on the plain
     step's state (tolerance: the fused kernel sums layer 0 in another order and applies
-    GELU to fp32 pre-activations, which the layer-by-layer form rounds to bf16)."""
+    the activation to fp32 pre-activations, which the layer-by-layer form rounds to bf16)."""
     from fip_amd.sim import Closure, Sim
     fine, coarse, bstd, zref = fine_coarse_case(384, 256, 1.0, 4, scenario=6, wet=0.5)
     coarse.h = coarse.h + np.float32(0.0)
     rng = np.random.default_rng(8)
-    cl = Closure(96, 64, 0, seed=2)
+    cl = Closure(96, 64, 0, seed=2, activation=act)
     cl.params = rand_params(rng, 0.3)
     a = Sim(coarse, mode="dense")
     a.set_run(1e9)
@@ -232,9 +236,10 @@ def test_fused_step_matches_layerwise_forward():
     cl.close()
 
 
+@pytest.mark.parametrize("act", ["gelu", "hsilu"])
 @pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("shape", [(96, 64), (50, 37), (130, 75)])
-def test_fused_kernel_forms_agree(shape, form):
+def test_fused_kernel_forms_agree(shape, form, act):
     """The persistent-tile form (weights in LDS, prefetched tiles) and the row-march form
     (58-column strips, LDS row rings) give the tile form's outputs and corrections bit for
     bit, on a multiple and non-multiples of the 16 x 16 tile and of the 58-column strip."""
@@ -245,7 +250,7 @@ def test_fused_kernel_forms_agree(shape, form):
     p = rand_params(rng, 0.3)
     cls, sims, outs = [], [], []
     for f in (0, form):
-        cl = Closure(Wc, Hc, 0, seed=2)
+        cl = Closure(Wc, Hc, 0, seed=2, activation=act)
         cl.params = p
         cl.set_kernel(f)
         s = Sim(coarse, mode="dense")
@@ -397,3 +402,24 @@ def test_mass_error_fallback_gives_the_plain_step():
         cl.close()
     for key in runs[0]:
         assert np.array_equal(runs[0][key].view(np.uint32), runs[1][key].view(np.uint32)), key
+
+
+def test_activation_is_checked_and_fixed_once_attached():
+    """An unknown activation is refused, and the activation cannot change while the
+    closure is attached (a captured step loop keeps the kernels it was captured with)."""
+    from fip_amd.sim import Closure, Sim, lib
+    with pytest.raises(ValueError):
+        Closure(16, 16, 0, seed=1, activation="relu")
+    fine, coarse, bstd, zref = fine_coarse_case(128, 128, 1.0, 4, scenario=2, wet=0.5)
+    cl = Closure(32, 32, 0, seed=1, activation="hsilu")
+    assert lib().fip_closure_set_activation(cl.c, 7) != 0
+    s = Sim(coarse, mode="dense")
+    s.set_run(1e9)
+    cl.attach(s, bstd, zref)
+    assert lib().fip_closure_set_activation(cl.c, 0) != 0
+    assert cl.activation == "hsilu"
+    s.simulate(3)
+    Closure.detach(s)
+    assert lib().fip_closure_set_activation(cl.c, 0) == 0 and cl.activation == "gelu"
+    s.close()
+    cl.close()
