@@ -407,7 +407,12 @@ int fip_closure_set_params(fip_closure* cl, const float* params) {
   if (!cl || !params) return fail(FIP_E_INVALID, "null argument");
   CHIP(hipSetDevice(cl->dev));
   CHIP(hipStreamSynchronize(cl->stream));
-  CHIP(hipMemcpy(cl->w, params, sizeof(float) * CL_LAYERS * CL_LAYER_PARAMS, hipMemcpyHostToDevice));
+  const size_t np = (size_t)CL_LAYERS * CL_LAYER_PARAMS;
+  CHIP(hipMemcpy(cl->w, params, sizeof(float) * np, hipMemcpyHostToDevice));
+  // new parameters start a new optimisation: Adam's moments and step count from zero
+  CHIP(hipMemset(cl->m, 0, sizeof(float) * np));
+  CHIP(hipMemset(cl->v, 0, sizeof(float) * np));
+  cl->t = 0;
   int rc = pack(cl);
   if (rc) return rc;
   CHIP(hipStreamSynchronize(cl->stream));

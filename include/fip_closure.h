@@ -54,6 +54,8 @@ int fip_closure_destroy(fip_closure* cl);
 int fip_closure_set_activation(fip_closure* cl, int act);
 int fip_closure_get_activation(fip_closure* cl, int* act);
 int fip_closure_get_params(fip_closure* cl, float* params);
+/* Sets the parameters and restarts the optimiser: Adam's moment estimates and step count
+ * are reset to zero. */
 int fip_closure_set_params(fip_closure* cl, const float* params);
 
 /* One training sample. `coarse` has just taken one plain step from a restricted state: its

@@ -350,21 +350,26 @@ def test_closure_ensemble_of_mixed_sizes_equals_single_domains():
 
 def test_adam_step_matches_oracle():
     """The optimiser step of training (cl_adam) equals the oracle's binary32 Adam bit for
-    bit over consecutive steps, moment estimates and bias corrections included."""
+    bit over consecutive steps, moment estimates and bias corrections included, and again
+    from fresh moments after new parameters are set."""
     from fip_amd.sim import Closure
     rng = np.random.default_rng(21)
     cl = Closure(16, 16, 0, seed=4)
     w = rand_params(rng, 0.3)
     cl.params = w
-    m = np.zeros_like(w)
-    v = np.zeros_like(w)
-    for t in range(1, 4):
-        g = np.where(co.live_mask(), rng.normal(0, 1e-2, co.NPARAMS), 0).astype(np.float32)
-        cl.adam_step(g, 1e-3)
-        w, m, v = co.adam(w, m, v, g, t, 1e-3)
-        got = cl.params
-        bad = np.count_nonzero(got.view(np.uint32) != w.view(np.uint32))
-        assert bad == 0, (t, bad, np.abs(got - w).max())
+    for restart in range(2):  # setting the parameters restarts the optimiser
+        if restart:
+            w = rand_params(rng, 0.3)
+            cl.params = w
+        m = np.zeros_like(w)
+        v = np.zeros_like(w)
+        for t in range(1, 4):
+            g = np.where(co.live_mask(), rng.normal(0, 1e-2, co.NPARAMS), 0).astype(np.float32)
+            cl.adam_step(g, 1e-3)
+            w, m, v = co.adam(w, m, v, g, t, 1e-3)
+            got = cl.params
+            bad = np.count_nonzero(got.view(np.uint32) != w.view(np.uint32))
+            assert bad == 0, (restart, t, bad, np.abs(got - w).max())
     cl.close()
 
 
