@@ -157,17 +157,20 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
             closure.attach_ensemble(e, bstd, zref, sat=sat)
         e.set_run([duration] * len(cases))
         e.simulate(200)  # graphs built, warm
+        t_warm = e.time(0)[0]  # every scenario of an ensemble here has the same fixed dt
         t0 = time.perf_counter()
-        e.simulate(10 ** 9)
+        # in stretches of 30 simulated minutes, so that a long run reports its progress
+        while e.simulate(int(round(1800.0 / cases[0].dt)))[0] > 0:
+            log(f"{cases[0].W}x{cases[0].H}...: {e.time(0)[0]:.0f} s simulated, {time.perf_counter() - t0:.1f} s")
         wall = time.perf_counter() - t0
-        return e, wall
+        # simulated hours per wall second over the timed steps (after the warm-up)
+        return e, len(cases) * (duration - t_warm) / 3600.0 / wall, wall
 
-    ef, wf = run(fines, None)
-    ep, wp = run(coarses, None)
+    ef, rf, wf = run(fines, None)
+    ep, rp, wp = run(coarses, None)
     n = len(scenarios)
     refs = [(block_mean(ef.field(k, "h")), block_mean(ef.field(k, "hmax"))) for k in range(n)]
     ef.close()
-    sim_h = n * duration / 3600.0
 
     def scores(e):
         rm, me, iou = [], [], []
@@ -179,8 +182,8 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
         return {"depth_rmse_m": round(float(np.mean(rm)), 4), "max_depth_err_m": round(float(np.mean(me)), 4),
                 "wet_iou": round(float(np.mean(iou)), 4)}
 
-    for name, wall in (("fine", wf), ("plain_coarse", wp)):
-        out[name + "_sim_h_per_wall_s"] = round(sim_h / wall, 3)
+    for name, rate, wall in (("fine", rf, wf), ("plain_coarse", rp, wp)):
+        out[name + "_sim_h_per_wall_s"] = round(rate, 3)
         out[name + "_wall_s"] = round(wall, 2)
     plain = scores(ep)
     ep.close()
@@ -188,7 +191,7 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
         cl.set_activation(act)
         if params is not None:
             cl.params = params
-        el, wl = run(coarses, cl, record=i == 0)
+        el, rl, wl = run(coarses, cl, record=i == 0)
         learned = scores(el)
         fb, mass = 0, 0.0
         for k in range(n):
@@ -196,8 +199,8 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
             fb += st["fallback"]
             mass = max(mass, st["max_rel_mass_error"])
         el.close()
-        res = {"activation": act, "learned_coarse_sim_h_per_wall_s": round(sim_h / wl, 3),
-               "learned_coarse_wall_s": round(wl, 2), "speedup_learned_vs_fine": round(wf / wl, 2),
+        res = {"activation": act, "learned_coarse_sim_h_per_wall_s": round(rl, 3),
+               "learned_coarse_wall_s": round(wl, 2), "speedup_learned_vs_fine": round(rl / rf, 2),
                **{"learned_" + k: v for k, v in learned.items()}, "fallback_scenarios": fb,
                "max_rel_mass_error": mass}
         if i == 0:
@@ -215,13 +218,14 @@ def evaluate(cl, scenarios, duration, variant="fast", W=1024, log=print, sat=2.6
 
 
 def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1500, batch=32, patch=128,
-               lr=1e-3, W=1024, eval_W=2048, activations=("gelu",), log=print):
+               lr=1e-3, W=1024, eval_W=2048, activations=("gelu",), eval_duration=None, log=print):
     """Generate samples (fine grids of W^2), train, evaluate on held-out scenarios with
     fine grids of eval_W^2, or eval_W[k]^2 for held-out scenario k (the network is
     convolutional, so it applies to any size). The
     held-out samples (four per held-out scenario, one a minute) come first in the store, the
     training samples (one every `sample_every` seconds of each training run) after them.
-    One network per activation, each trained from the same initial weights on the same
+    The held-out runs last eval_duration seconds (default: duration). One network per
+    activation, each trained from the same initial weights on the same
     samples; the first gives the top-level figures, each other one a sub-object under its
     name (training figures and the held-out evaluation)."""
     from .sim import Closure
@@ -251,7 +255,8 @@ def experiment(n_train=16, n_test=8, duration=1800.0, sample_every=60.0, steps=1
     res = dict(runs[0])
     for act, r in zip(activations[1:], runs[1:]):
         res[act] = r
-    res["eval"] = evaluate(cl, range(1000, 1000 + n_test), duration, W=eval_W, log=log, nets=nets)
+    res["eval"] = evaluate(cl, range(1000, 1000 + n_test), eval_duration or duration, W=eval_W, log=log,
+                           nets=nets)
     cl.close()
     return res
 

@@ -27,7 +27,9 @@ int fip_ens_create(const fip_desc* descs, int n, fip_ens** out);
 int fip_ens_destroy(fip_ens* e);
 /* The BC list of scenario s (fip_bc as for fip_bc_generate). */
 int fip_ens_set_bcs(fip_ens* e, int s, const fip_bc* bcs, int n);
-/* Target duration of every scenario (n values); resets nothing else. */
+/* Target duration of every scenario (n values); resets nothing else. A scenario that has
+ * reached its target stays finished; a later call moves the targets of those still
+ * running. */
 int fip_ens_set_run(fip_ens* e, const double* target_durations);
 /* Runs up to nsteps ensemble steps, or until every scenario reached its target.
  * *active receives the number of scenarios still running. */
