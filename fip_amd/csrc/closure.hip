@@ -487,35 +487,39 @@ extern "C" __global__ void cl_adam(float* w, float* m, float* v, const float* g,
   w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// The correction of cell (x, y)'s east and north faces by the network output o ([H][W][2]):
-// on interior faces q += tanh(o) * bound * |q| unless |o| > sat (counted, the face keeps
-// the plain step) or the domain fell back (off). Returns the saturated faces. It also
-// returns the cell's mass-balance terms of the uncorrected state (see cl_mass_check): v,
-// the depth of a cell of the box [2, W-3] x [2, H-3], and f, the discharges the step
-// computed through the box's faces (+ east and north, - west and south), each read by
-// the thread that corrects that face, before its correction.
-__device__ __forceinline__ unsigned correct_cell(const fip_grid& g, int cur, const float* o, int x, int y,
-                                                 float bound, float sat, bool off, double& v, double& f) {
-  const long p = (long)y * g.W + x, i = (long)y * g.P + x;
+// One cell's state as the correction reads it: depth, east and north face discharges and
+// the two network outputs.
+struct ClCell {
+  float h, qx, qy, ox, oy;
+};
+
+// The correction of cell (x, y)'s east and north faces (c: the cell's values, q the
+// discharges to write) by the network outputs: on interior faces q += tanh(o) * bound * |q|
+// unless |o| > sat (counted, the face keeps the plain step) or the domain fell back (off).
+// Returns the saturated faces. It also returns the cell's mass-balance terms of the
+// uncorrected state (see mass_check): v, the depth of a cell of the box [2, W-3] x
+// [2, H-3], and f, the discharges the step computed through the box's faces (+ east and
+// north, - west and south), each read by the thread that corrects that face, before its
+// correction.
+__device__ __forceinline__ unsigned correct_cell(const fip_grid& g, const ClCell& c, float* qx, float* qy, int x,
+                                                 int y, float bound, float sat, bool off, double& v, double& f) {
   unsigned nsat = 0;
   v = 0.0;
   f = 0.0;
-  if (x >= 2 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) v = (double)g.h[cur][i];
+  if (x >= 2 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) v = (double)c.h;
   if (x >= 1 && x <= g.W - 3 && y >= 2 && y <= g.H - 3) {
-    const float ox = o[p * 2];
-    const float q = g.qx[cur][i];
+    const float q = c.qx;
     if (x == g.W - 3) f += (double)q;
     if (x == 1) f -= (double)q;
-    if (fabsf(ox) > sat) nsat++;
-    else if (!off) g.qx[cur][i] = q + tanhf(ox) * bound * fabsf(q);
+    if (fabsf(c.ox) > sat) nsat++;
+    else if (!off) *qx = q + tanhf(c.ox) * bound * fabsf(q);
   }
   if (x >= 2 && x <= g.W - 3 && y >= 1 && y <= g.H - 3) {
-    const float oy = o[p * 2 + 1];
-    const float q = g.qy[cur][i];
+    const float q = c.qy;
     if (y == g.H - 3) f += (double)q;
     if (y == 1) f -= (double)q;
-    if (fabsf(oy) > sat) nsat++;
-    else if (!off) g.qy[cur][i] = q + tanhf(oy) * bound * fabsf(q);
+    if (fabsf(c.oy) > sat) nsat++;
+    else if (!off) *qy = q + tanhf(c.oy) * bound * fabsf(q);
   }
   return nsat;
 }
@@ -580,18 +584,33 @@ __device__ __forceinline__ unsigned wave_sum_parts(const unsigned* part, const d
 __device__ __forceinline__ unsigned correct_rows(const fip_grid& g, int cur, const float* o, float bound, float sat,
                                                  bool off, double& v, double& f) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y0 = blockIdx.y * CL_CORRECT_ROWS;
   unsigned nsat = 0;
   v = 0.0;
   f = 0.0;
-  if (x < g.W)
-    for (int r = 0; r < CL_CORRECT_ROWS; r++) {
-      const int y = blockIdx.y * CL_CORRECT_ROWS + r;
-      if (y >= g.H) break;
-      double cv, cf;
-      nsat += correct_cell(g, cur, o, x, y, bound, sat, off, cv, cf);
-      v += cv;
-      f += cf;
+  if (x >= g.W) return 0;
+  // every load of the thread's rows first: the rows are independent, so no load waits for
+  // the stores of the rows before it
+  ClCell c[CL_CORRECT_ROWS];
+#pragma unroll
+  for (int r = 0; r < CL_CORRECT_ROWS; r++) {
+    const int y = y0 + r;
+    if (y < g.H) {
+      const long i = (long)y * g.P + x;
+      const float2 ov = *(const float2*)(o + ((long)y * g.W + x) * 2);
+      c[r] = ClCell{g.h[cur][i], g.qx[cur][i], g.qy[cur][i], ov.x, ov.y};
     }
+  }
+#pragma unroll
+  for (int r = 0; r < CL_CORRECT_ROWS; r++) {
+    const int y = y0 + r;
+    if (y >= g.H) break;
+    const long i = (long)y * g.P + x;
+    double cv, cf;
+    nsat += correct_cell(g, c[r], g.qx[cur] + i, g.qy[cur] + i, x, y, bound, sat, off, cv, cf);
+    v += cv;
+    f += cf;
+  }
   return nsat;
 }
 
