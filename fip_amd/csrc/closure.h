@@ -40,8 +40,21 @@ struct ClMass {
   int fallback;      // set once the relative mass error exceeded the limit
   unsigned nsat;     // faces that kept the plain step because the network saturated
   unsigned steps;
+  double t_pend;     // duration after the corrected step whose check is pending
+  int pending;       // the correction wrote its slots; the check has not run yet
+  int pad;
 };
 
+
+// The pending mass checks that the row-march network kernel runs (one extra work-group
+// per domain): each domain's nparts slots in part and mpart, its ClMass in ms.
+struct ClMassCheck {
+  ClMass* ms;
+  const unsigned* part;
+  const double* mpart;
+  int nparts;
+  double limit;
+};
 
 // One coarse domain of the fused inference step.
 struct ClDom {
@@ -56,10 +69,11 @@ struct ClDom {
 // Network outputs o of every cell of ndom domains (domain d's [H][W][2] image at pixel
 // offset dom[d].o_off; the domains may differ in size, W x H bounds them all). form 2:
 // row march over the work items of cl_fused_plan (items, nitems), 1: persistent tiles,
-// 0: one work-group per tile. act: CL_ACT_GELU or CL_ACT_HSILU.
+// 0: one work-group per tile. act: CL_ACT_GELU or CL_ACT_HSILU. mc (form 2, or null): the
+// pending mass checks of the step before, run by one more work-group per domain.
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
                            const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems,
-                           int act);
+                           int act, const ClMassCheck* mc);
 // The row march's work items (domain, first column, first row, end row) for domains of
 // sizes W[d] x H[d] on a device of `cus` compute units: strips of the kernel's width and
 // spans of one length, chosen so that the items (one work-group per compute unit at a
@@ -98,12 +112,12 @@ hipError_t cl_adam_launch(float* w, float* m, float* v, const float* g, int nlay
 inline int cl_correct_parts(int W, int H) { return (W + 255) / 256 * ((H + CL_CORRECT_ROWS - 1) / CL_CORRECT_ROWS); }
 hipError_t cl_correct_launch(fip_grid g, int par, const float* o, float bound, float sat, ClMass* ms,
                              unsigned* part, double* mpart, hipStream_t st);
-hipError_t cl_mass_check_launch(fip_grid g, int par, const double* mpart, ClMass* ms, double limit,
-                                const unsigned* part, int nparts, hipStream_t st);
+hipError_t cl_mass_check_launch(fip_grid g, const double* mpart, ClMass* ms, double limit, const unsigned* part,
+                                int nparts, hipStream_t st);
 // (ensembles: dom[s].o_off places scenario s's outputs in o; W x H bounds every scenario,
 // and a scenario's work-groups past its own size write empty slots)
 hipError_t cl_correct_ens_launch(const fip_ens_scn* scn, const ClDom* dom, int par, const float* o, float bound,
                                  float sat, ClMass* ms, unsigned* part, double* mpart, int W, int H, int n,
                                  hipStream_t st);
-hipError_t cl_mass_check_ens_launch(const fip_ens_scn* scn, int n, int par, const double* mpart, ClMass* ms,
-                                    double limit, const unsigned* part, int nparts, hipStream_t st);
+hipError_t cl_mass_check_ens_launch(const fip_ens_scn* scn, int n, const double* mpart, ClMass* ms, double limit,
+                                    const unsigned* part, int nparts, hipStream_t st);

@@ -621,6 +621,10 @@ extern "C" __global__ __launch_bounds__(256) void cl_correct(fip_grid g, int par
   const unsigned nsat = correct_rows(g, par ^ 1, o, bound, sat, ms->fallback != 0, v, f);
   const long slot = (long)blockIdx.y * gridDim.x + blockIdx.x;
   block_count(part + slot, mpart + 2 * slot, nsat, v, f);
+  if (slot == 0 && threadIdx.x == 0) {  // the step's check is due (pending_check)
+    ms->t_pend = g.ctl->duration;
+    ms->pending = 1;
+  }
 }
 
 // Mass balance of a coarse domain without no_data cells, checked after each step before
@@ -630,11 +634,11 @@ extern "C" __global__ __launch_bounds__(256) void cl_correct(fip_grid g, int par
 // with the face discharges the step computed (stored and not yet corrected). cl_correct
 // adds the box's depths and the bracket per work-group (fp64, correct_cell); the check
 // adds those slots in a fixed order, compares, and sets the fallback flag when the
-// relative error exceeds `limit`.
-__device__ __forceinline__ void mass_check(const fip_grid& g, double vsum, double fsum, ClMass& m, double limit) {
+// relative error exceeds `limit`. t: the duration after the step.
+__device__ __forceinline__ void mass_check(const fip_grid& g, double t, double vsum, double fsum, ClMass& m,
+                                           double limit) {
   const double dx = g.dx;
   const double vol = vsum * dx * dx;
-  const double t = g.ctl->duration;
   if (m.valid) {
     const double pred = m.v_prev - (t - m.t_prev) * dx * fsum;
     const double rel = fabs(vol - pred) / (vol > 1e-30 ? vol : 1e-30);
@@ -648,16 +652,25 @@ __device__ __forceinline__ void mass_check(const fip_grid& g, double vsum, doubl
   m.steps = m.steps + 1;
 }
 
-// One wavefront: adds the saturation counts and mass terms of the step (nparts slots),
-// then the check.
-extern "C" __global__ void cl_mass_check(fip_grid g, int par, const double* mpart, ClMass* ms, double limit,
-                                         const unsigned* part, int nparts) {
-  if (g.ctl->halt_req[par ^ 1]) return;
+// The check of the last corrected step, if it has not run yet (m.pending, set by the
+// correction): one wavefront adds the saturation counts and mass terms of its nparts
+// slots, then the check. It runs before the next correction, which reads the fallback
+// flag: in the next step's network kernel (cl_fused_r's extra work-groups), or as
+// cl_mass_check(_ens) after the correction (the tile forms) and before a status read.
+__device__ __forceinline__ void pending_check(const fip_grid& g, const unsigned* part, const double* mpart,
+                                              int nparts, ClMass& m, double limit) {
+  if (threadIdx.x >= 64 || !m.pending) return;
   double v, f;
   const unsigned t = wave_sum_parts(part, mpart, nparts, v, f);
   if (threadIdx.x != 0) return;
-  ms->nsat += t;
-  mass_check(g, v, f, *ms, limit);
+  m.nsat += t;
+  mass_check(g, m.t_pend, v, f, m, limit);
+  m.pending = 0;
+}
+
+extern "C" __global__ void cl_mass_check(fip_grid g, const double* mpart, ClMass* ms, double limit,
+                                         const unsigned* part, int nparts) {
+  pending_check(g, part, mpart, nparts, *ms, limit);
 }
 
 // Ensemble forms of cl_correct / cl_mass_check: blockIdx.z (cl_mass_check_ens:
@@ -675,18 +688,16 @@ extern "C" __global__ __launch_bounds__(256) void cl_correct_ens(const fip_ens_s
   const long nparts = (long)gridDim.x * gridDim.y;
   const long slot = s * nparts + (long)blockIdx.y * gridDim.x + blockIdx.x;
   block_count(part + slot, mpart + 2 * slot, nsat, v, f);
+  if (slot == s * nparts && threadIdx.x == 0) {  // the step's check is due (pending_check)
+    ms[s].t_pend = g.ctl->duration;
+    ms[s].pending = 1;
+  }
 }
 
-extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, int n, int par, const double* mpart,
-                                             ClMass* ms, double limit, const unsigned* part, int nparts) {
+extern "C" __global__ void cl_mass_check_ens(const fip_ens_scn* scn, const double* mpart, ClMass* ms, double limit,
+                                             const unsigned* part, int nparts) {
   const int s = blockIdx.x;
-  const fip_grid& g = scn[s].g;
-  if (g.ctl->halt_req[par ^ 1]) return;
-  double v, f;
-  const unsigned t = wave_sum_parts(part + (long)s * nparts, mpart + 2L * s * nparts, nparts, v, f);
-  if (threadIdx.x != 0) return;
-  ms[s].nsat += t;
-  mass_check(g, v, f, ms[s], limit);
+  pending_check(scn[s].g, part + (long)s * nparts, mpart + 2L * s * nparts, nparts, ms[s], limit);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1375,14 +1386,23 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
   }
 }
 
+// Work-groups nitems.. (one per domain, when mc is given) run the pending mass check of the
+// step before (pending_check), which the correction of this step waits for.
 template <int ACT>
 __global__ __launch_bounds__(1024, 1) void cl_fused_r(const ClDom* __restrict__ dom, int par,
                                                              const u16* wf, const u16* wl1, const float* w,
-                                                             float* o_out, const int4* __restrict__ items) {
+                                                             float* o_out, const int4* __restrict__ items,
+                                                             int nitems, ClMassCheck mc) {
   __shared__ __attribute__((aligned(16))) u16 fring[RROWS * RFW * 8];
   __shared__ __attribute__((aligned(16))) u16 ring1[RROWS * RPX * FPS];
   __shared__ __attribute__((aligned(16))) u16 ring2[RROWS * RPX * FPS];
   __shared__ __attribute__((aligned(16))) u16 ring3[RROWS * RPX * FPS];
+  if ((int)blockIdx.x >= nitems) {
+    const int d = blockIdx.x - nitems;
+    pending_check(dom[d].g, mc.part + (long)d * mc.nparts, mc.mpart + 2L * d * mc.nparts, mc.nparts, mc.ms[d],
+                  mc.limit);
+    return;
+  }
   const int4 it = items[blockIdx.x];  // (domain, first column, first row, end row)
   const int d = uni(it.x);
   const ClDom& D = dom[d];
@@ -1530,16 +1550,16 @@ CL_WRAP(cl_correct, (fip_grid g, int par, const float* o, float bound, float sat
                      double* mpart, hipStream_t st),
         P(g, par, o, bound, sat, ms, part, mpart),
         dim3((g.W + 255) / 256, (g.H + CL_CORRECT_ROWS - 1) / CL_CORRECT_ROWS), dim3(256))
-CL_WRAP(cl_mass_check, (fip_grid g, int par, const double* mpart, ClMass* ms, double limit, const unsigned* part,
-                        int nparts, hipStream_t st),
-        P(g, par, mpart, ms, limit, part, nparts), dim3(1), dim3(64))
+CL_WRAP(cl_mass_check, (fip_grid g, const double* mpart, ClMass* ms, double limit, const unsigned* part, int nparts,
+                        hipStream_t st),
+        P(g, mpart, ms, limit, part, nparts), dim3(1), dim3(64))
 CL_WRAP(cl_correct_ens, (const fip_ens_scn* scn, const ClDom* dom, int par, const float* o, float bound, float sat,
                          ClMass* ms, unsigned* part, double* mpart, int W, int H, int n, hipStream_t st),
         P(scn, dom, par, o, bound, sat, ms, part, mpart),
         dim3((W + 255) / 256, (H + CL_CORRECT_ROWS - 1) / CL_CORRECT_ROWS, n), dim3(256))
-CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, int par, const double* mpart, ClMass* ms,
-                            double limit, const unsigned* part, int nparts, hipStream_t st),
-        P(scn, n, par, mpart, ms, limit, part, nparts), dim3(n), dim3(64))
+CL_WRAP(cl_mass_check_ens, (const fip_ens_scn* scn, int n, const double* mpart, ClMass* ms, double limit,
+                            const unsigned* part, int nparts, hipStream_t st),
+        P(scn, mpart, ms, limit, part, nparts), dim3(n), dim3(64))
 CL_WRAP(cl_pack_l1, (const float* w, u16* wl1, hipStream_t st), P(w, wl1), dim3((CL_TAPS * 8 * CL_C + 255) / 256),
         dim3(256))
 
@@ -1571,7 +1591,8 @@ std::vector<int4> cl_fused_plan(const int* W, const int* H, int ndom, int cus) {
 
 template <int ACT>
 hipError_t fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
-                        const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems) {
+                        const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems,
+                        const ClMassCheck* mc) {
   const int tx = (W + FT - 1) / FT, ty = (H + FT - 1) / FT;
   if (form == 0) {
     hipLaunchKernelGGL(cl_fused<ACT>, dim3(tx, ty, ndom), dim3(512), 0, st, dom, par, wf, wl1, w, o);
@@ -1579,7 +1600,9 @@ hipError_t fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const
   }
   if (form == 2) {
     if (!items || nitems < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(cl_fused_r<ACT>, dim3(nitems), dim3(64 * RNW), 0, st, dom, par, wf, wl1, w, o, items);
+    const ClMassCheck m = mc ? *mc : ClMassCheck{};
+    hipLaunchKernelGGL(cl_fused_r<ACT>, dim3(nitems + (mc ? ndom : 0)), dim3(64 * RNW), 0, st, dom, par, wf, wl1,
+                       w, o, items, nitems, m);
     return hipGetLastError();
   }
   static int cus[64] = {};
@@ -1601,8 +1624,8 @@ hipError_t fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const
 
 hipError_t cl_fused_launch(const ClDom* dom, int ndom, int W, int H, int par, const u16* wf, const u16* wl1,
                            const float* w, float* o, hipStream_t st, int form, const int4* items, int nitems,
-                           int act) {
+                           int act, const ClMassCheck* mc) {
   if (act == CL_ACT_HSILU)
-    return fused_launch<CL_ACT_HSILU>(dom, ndom, W, H, par, wf, wl1, w, o, st, form, items, nitems);
-  return fused_launch<CL_ACT_GELU>(dom, ndom, W, H, par, wf, wl1, w, o, st, form, items, nitems);
+    return fused_launch<CL_ACT_HSILU>(dom, ndom, W, H, par, wf, wl1, w, o, st, form, items, nitems, mc);
+  return fused_launch<CL_ACT_GELU>(dom, ndom, W, H, par, wf, wl1, w, o, st, form, items, nitems, mc);
 }

@@ -145,9 +145,12 @@ int device_cus(int dev) {
   return n;
 }
 
-// the row march's work items for domains of the given sizes, uploaded to *items
-int upload_plan(const std::vector<int>& W, const std::vector<int>& H, int dev, int4** items, int* nitems) {
-  const std::vector<int4> it = cl_fused_plan(W.data(), H.data(), (int)W.size(), device_cus(dev));
+// the row march's work items for domains of the given sizes, uploaded to *items; `extra`
+// compute units are left to the kernel's mass-check work-groups
+int upload_plan(const std::vector<int>& W, const std::vector<int>& H, int dev, int4** items, int* nitems,
+                int extra = 0) {
+  const std::vector<int4> it =
+      cl_fused_plan(W.data(), H.data(), (int)W.size(), std::max(1, device_cus(dev) - extra));
   int rc = dalloc(items, it.size());
   if (rc) return rc;
   CHIP(hipMemcpy(*items, it.data(), it.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -585,15 +588,18 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
   ClDom hd{g, a->bstd, 0, zref, 0};
   if ((rc = dalloc(&a->dom, 1))) return rc;
   CHIP(hipMemcpy(a->dom, &hd, sizeof hd, hipMemcpyHostToDevice));
-  if ((rc = upload_plan({g.W}, {g.H}, cl->dev, &a->items, &a->nitems))) return rc;
+  if ((rc = upload_plan({g.W}, {g.H}, cl->dev, &a->items, &a->nitems, 1))) return rc;
   Attachment* ap = a.get();
   cl->att.push_back(std::move(a));
   fip_internal::set_post_hook(coarse, [cl, ap](hipStream_t st, int par) -> int {
     const fip_grid& g = fip_internal::grid(ap->sim);
+    // the row march runs the step before's mass check; the tile forms check after the correction
+    const bool march = cl->fused_form == 2;
+    const ClMassCheck mc{ap->ms, ap->part, ap->mpart, ap->nparts, ap->limit};
     CHIP(cl_fused_launch(ap->dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form, ap->items,
-                         ap->nitems, cl->act));
+                         ap->nitems, cl->act, march ? &mc : nullptr));
     CHIP(cl_correct_launch(g, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, st));
-    CHIP(cl_mass_check_launch(g, par, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
+    if (!march) CHIP(cl_mass_check_launch(g, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
   });
   return FIP_OK;
@@ -640,16 +646,18 @@ int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_
     hd[s2] = ClDom{fip_internal::grid(fip_internal::ens_member(ens, s2)), a->bstd + off[s2], off[s2], zref[s2], 0};
   if ((rc = dalloc(&a->dom, (size_t)n))) return rc;
   CHIP(hipMemcpy(a->dom, hd.data(), n * sizeof(ClDom), hipMemcpyHostToDevice));
-  if ((rc = upload_plan(Ws, Hs, cl->dev, &a->items, &a->nitems))) return rc;
+  if ((rc = upload_plan(Ws, Hs, cl->dev, &a->items, &a->nitems, n))) return rc;
   Attachment* ap = a.get();
   cl->att.push_back(std::move(a));
   fip_internal::ens_set_post_hook(ens, [cl, ap, n](hipStream_t st, int par) -> int {
     const fip_ens_scn* scn = fip_internal::ens_scn(ap->ens);
+    const bool march = cl->fused_form == 2;
+    const ClMassCheck mc{ap->ms, ap->part, ap->mpart, ap->nparts, ap->limit};
     CHIP(cl_fused_launch(ap->dom, n, ap->wmax, ap->hmax, par, cl->wf, cl->wl1, cl->w, ap->o, st, cl->fused_form,
-                         ap->items, ap->nitems, cl->act));
+                         ap->items, ap->nitems, cl->act, march ? &mc : nullptr));
     CHIP(cl_correct_ens_launch(scn, ap->dom, par, ap->o, kBound, ap->sat, ap->ms, ap->part, ap->mpart, ap->wmax,
                                ap->hmax, n, st));
-    CHIP(cl_mass_check_ens_launch(scn, n, par, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
+    if (!march) CHIP(cl_mass_check_ens_launch(scn, n, ap->mpart, ap->ms, ap->limit, ap->part, ap->nparts, st));
     return FIP_OK;
   });
   return FIP_OK;
@@ -674,6 +682,13 @@ int fip_closure_status(fip_closure* cl, fip_sim* coarse, fip_ens* ens, int s, in
   if (!a) return fail(FIP_E_STATE, "not attached");
   if (ens && (s < 0 || s >= fip_internal::ens_count(ens))) return fail(FIP_E_INVALID, "scenario out of range");
   CHIP(hipSetDevice(cl->dev));
+  CHIP(hipDeviceSynchronize());
+  // the check of the last corrected step, if the next step's network kernel has not run it
+  if (ens)
+    CHIP(cl_mass_check_ens_launch(fip_internal::ens_scn(ens), fip_internal::ens_count(ens), a->mpart, a->ms, a->limit,
+                                  a->part, a->nparts, nullptr));
+  else
+    CHIP(cl_mass_check_launch(fip_internal::grid(coarse), a->mpart, a->ms, a->limit, a->part, a->nparts, nullptr));
   CHIP(hipDeviceSynchronize());
   ClMass m;
   CHIP(hipMemcpy(&m, a->ms + (ens ? s : 0), sizeof m, hipMemcpyDeviceToHost));
@@ -778,7 +793,7 @@ int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed
   // par such that par ^ 1 is the current side and the halt flag read is that of no step
   const int par = fip_internal::cur_side(coarse) ^ 1;
   hipError_t e = cl_fused_launch(dom, 1, g.W, g.H, par, cl->wf, cl->wl1, cl->w, o, nullptr, cl->fused_form, items,
-                                 nitems, cl->act);
+                                 nitems, cl->act, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(out, o, spx * 2 * sizeof(float), hipMemcpyDeviceToHost);
   hipFree(bstd); hipFree(o); hipFree(dom); hipFree(items);
