@@ -81,8 +81,9 @@ int fip_closure_adam_step(fip_closure* cl, const float* grad, float lr);
 int fip_closure_eval(fip_closure* cl, int first, int n, double* mse_plain, double* mse_learned);
 
 /* Inference: after every step of fip_simulate on `coarse`, its face discharges are
- * corrected (faces where |o| > sat keep the plain step), and the mass balance is checked;
- * a relative error above mass_limit switches the domain to the plain step for good. */
+ * corrected (faces where |o| > sat keep the plain step), and the mass balance is checked
+ * before the next correction; a relative error above mass_limit switches the domain to
+ * the plain step for good. */
 int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, float zref,
                        float sat, double mass_limit);
 /* The same for every scenario of an ensemble; the scenarios may differ in size. bed_std
@@ -91,7 +92,8 @@ int fip_closure_attach(fip_closure* cl, fip_sim* coarse, const float* bed_std, f
 int fip_closure_attach_ensemble(fip_closure* cl, fip_ens* ens, const float* bed_std,
                                 const float* zref, float sat, double mass_limit);
 int fip_closure_detach(fip_sim* coarse);
-/* State of an attached domain (scenario s of an ensemble, or s = 0 with ens = NULL). */
+/* State of an attached domain (scenario s of an ensemble, or s = 0 with ens = NULL),
+ * the mass check of the last step included. */
 int fip_closure_status(fip_closure* cl, fip_sim* coarse, fip_ens* ens, int s, int* fallback,
                        unsigned* saturated_faces, double* max_rel_mass_error);
 
