@@ -820,6 +820,9 @@ int fip_closure_set_kernel(fip_closure* cl, int form) {
   if (!cl) return fail(FIP_E_INVALID, "null argument");
   if (form < 0 || form > 2)
     return fail(FIP_E_INVALID, "kernel form must be 0 (per tile), 1 (persistent tiles) or 2 (row march)");
+  // the row march runs a step's mass check in the next step's launch, the tile forms after
+  // the correction: a change between two steps would skip a check
+  if (!cl->att.empty()) return fail(FIP_E_INVALID, "set the kernel form before attaching the closure");
   cl->fused_form = form;
   return FIP_OK;
 }

@@ -124,8 +124,8 @@ int fip_closure_fused_outputs(fip_closure* cl, fip_sim* coarse, const float* bed
 /* Launch form of the fused inference kernel: 2 (default) = row march (strips of 58 columns
  * marched along their rows, each layer's last rows in LDS rings); 1 = persistent, one work-group
  * per CU over runs of 16 x 16 tiles with the weights staged in LDS; 0 = one work-group per
- * 16 x 16 tile. All give bit-identical outputs. Set it before fip_closure_attach*: a step
- * loop captured in a graph keeps the form it was captured with. */
+ * 16 x 16 tile. All give bit-identical outputs. It is refused while the closure is
+ * attached. */
 int fip_closure_set_kernel(fip_closure* cl, int form);
 /* D = A B of one bf16 matrix-core instruction (A 16x32, B 32x16 row-major bf16 bits,
  * D 16x16 fp32): checks the operand layout the convolutions assume. */

@@ -410,8 +410,9 @@ def test_mass_error_fallback_gives_the_plain_step():
 
 
 def test_activation_is_checked_and_fixed_once_attached():
-    """An unknown activation is refused, and the activation cannot change while the
-    closure is attached (a captured step loop keeps the kernels it was captured with)."""
+    """An unknown activation is refused, and neither the activation nor the kernel form
+    can change while the closure is attached (a captured step loop keeps the kernels it
+    was captured with, and the forms run the mass check at different points)."""
     from fip_amd.sim import Closure, Sim, lib
     with pytest.raises(ValueError):
         Closure(16, 16, 0, seed=1, activation="relu")
@@ -422,6 +423,7 @@ def test_activation_is_checked_and_fixed_once_attached():
     s.set_run(1e9)
     cl.attach(s, bstd, zref)
     assert lib().fip_closure_set_activation(cl.c, 0) != 0
+    assert lib().fip_closure_set_kernel(cl.c, 0) != 0
     assert cl.activation == "hsilu"
     s.simulate(3)
     Closure.detach(s)
