@@ -1294,12 +1294,16 @@ __device__ __forceinline__ void rm_march(const u16* wp, const float* bp, const u
   f32x4 acc0[MT], acc1[MT];  // a pair between its matrix instructions and its epilogue
   const auto mfma_pair = [&](auto slot) {
     constexpr int S = decltype(slot)::value;
+    // a wavefront issuing matrix work goes before the epilogue wavefronts of its SIMD,
+    // which fill the cycles the matrix pipe leaves
+    __builtin_amdgcn_s_setprio(2);
 #pragma unroll
     for (int m = 0; m < MT; m++) acc0[m] = acc1[m] = bias[m];
     if constexpr (L == 0)
       rm_mfma2_l0<S>(in, l0off, tap8, a, acc0, acc1);
     else
       rm_mfma2<MT, S>(inl, a, acc0, acc1);
+    __builtin_amdgcn_s_setprio(0);
   };
   const auto epi_pair = [&](auto slot, int k) {
     constexpr int S = decltype(slot)::value;
